@@ -1,0 +1,186 @@
+"""Benchmark: batched FP + matched-adjoint step on clinical-size fan-beam slices.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices 64]
+
+Workload: 512x512 fp32 images (0.7 mm pixels), SID 541 mm, SDD 949 mm, 888 equiangular
+bins (1.0239 mm at the detector), 984 views over 360 deg; seeded ellipse phantoms plus
+N(0, 1e-4) noise.  One step = forward projection of a batch of slices followed by the
+matched adjoint applied to the residual (the forward/backward of a projection loss).
+
+Prints one JSON line.  value = GUPS of the step = 2 * slices * nX*nY*nViews / t (FP and
+its adjoint each count nX*nY*nViews voxel-view updates per slice).  Also reported:
+FP / BP / FBP GUPS on the batch, ms/slice batched vs the plain per-slice drop-in
+calls, the dominant kernel's roofline figures and the FP64 CPU oracle timed on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "GUPS for FP & BP at 512²×984v; batched fwd+adjoint ms/slice vs plain"
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78,643 GB/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (slices shard across GPUs "
+              "with no communication; see the sharding notes in DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from fanbeamctrecon_tensorflowop_amd import workloads
+    from fanbeamctrecon_tensorflowop_amd.plan import FanBeamPlan
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    S = args.slices
+    p, a = workloads.clinical_scan(n_slices=S)
+    plan = FanBeamPlan(p, a)
+    nX, nY, nV, nC = plan.nx, plan.ny, plan.n_views, plan.n_cols
+    U = nX * nY * nV                      # voxel-view updates per slice per operator
+    samples = plan.fp_samples             # FP samples per slice
+
+    log("bench: building %d phantoms" % S)
+    base = workloads.clinical_phantom(min(S, 8))
+    img = np.concatenate([np.roll(base, 7 * k, axis=2) for k in range((S + 7) // 8)])[:S]
+    x = torch.from_numpy(np.ascontiguousarray(img)).to(dev)
+    target = plan.fp(x) * 1.01
+    torch.cuda.synchronize()
+
+    def step():
+        y = plan.fp(x)
+        return plan.fp_adjoint(y - target)
+
+    log("bench: warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    log("bench: timing %d steps" % args.steps)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    gups = 2.0 * S * U / (ms_step * 1e-3) / 1e9
+
+    def time_op(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    reps = max(3, args.steps // 2)
+    log("bench: per-operator timings")
+    sino = target.contiguous()
+    ms_fp = time_op(lambda: plan.fp(x), reps)
+    ms_fpa = time_op(lambda: plan.fp_adjoint(sino), reps)
+    ms_bp = time_op(lambda: plan.bp(sino), reps)
+    ms_fbp = time_op(lambda: plan.fbp(sino), reps)
+
+    log("bench: plain per-slice drop-in")
+    xs = [x[s:s + 1].contiguous() for s in range(S)]
+    ts = [target[s:s + 1].contiguous() for s in range(S)]
+
+    def plain():
+        for s in range(S):
+            y = plan.fp(xs[s])
+            plan.fp_adjoint(y - ts[s])
+
+    ms_plain = time_op(plain, max(2, reps // 2))
+
+    # dominant kernel: FP ray march (one launch per fp call)
+    fp_bytes = 4 * 4 * samples * S       # 4 taps x 4 B gathered per sample (algorithmic)
+    achieved = fp_bytes / (ms_fp * 1e-3) / 1e9
+    hbm_bytes = 4 * (nX * nY + nV * nC) * S  # compulsory: image in + sinogram out
+    out = {
+        "metric": METRIC,
+        "value": round(gups, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "clinical fan-beam FP + matched adjoint, %d slices of 512x512, 984 views x 888 bins" % S,
+            "slices": S, "image": [nY, nX], "views": nV, "bins": nC, "storage": "f32",
+        },
+        "fp_gups": round(S * U / (ms_fp * 1e-3) / 1e9, 3),
+        "fp_adjoint_gups": round(S * U / (ms_fpa * 1e-3) / 1e9, 3),
+        "bp_gups": round(S * U / (ms_bp * 1e-3) / 1e9, 3),
+        "fbp_gups": round(S * U / (ms_fbp * 1e-3) / 1e9, 3),
+        "ms_per_slice_batched": round(ms_step / S, 4),
+        "ms_per_slice_plain": round(ms_plain / S, 4),
+        "batched_vs_plain": round(ms_plain / ms_step, 3),
+        "wall_s": round(wall, 3),
+        "roofline": {
+            "kernel": "fp_kernel",
+            "bound": "lds",
+            "achieved": round(achieved, 1),
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / LDS_PEAK_GBS, 4),
+            "traffic": None,
+            "launch_ms": round(ms_fp, 4),
+            "algorithmic_bytes_per_launch": fp_bytes,
+            "compulsory_hbm_bytes_per_launch": hbm_bytes,
+            "hbm_frac_compulsory": round(hbm_bytes / (ms_fp * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        log("bench: CPU oracle baseline (1 slice FP + adjoint)")
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as orc
+        o = orc.Oracle(workloads.clinical_scan()[0], a)
+        nthr = orc.default_threads()
+        one = img[:1]
+        t0 = time.perf_counter()
+        yo = o.fp(one, nthreads=nthr)
+        o.fp_adjoint(yo, nthreads=nthr)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(2.0 * U / tc / 1e9, 4), "unit": "GUPS", "cores": nthr, "kind": "port",
+            "sample": "FP64 oracle FP + matched adjoint of 1 of the %d slices (%.1f s)" % (S, tc),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

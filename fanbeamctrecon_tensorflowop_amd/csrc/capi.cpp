@@ -1,0 +1,132 @@
+// capi.cpp -- execute entry points of the C ABI: argument validation and dispatch.
+//
+// The reference op wrappers (FP/transform.cc:93-117 and siblings) trusted their
+// inputs (DCHECKs compile out, :97,105) and allocated / freed device scratch on
+// every call (FP/transform.cu:198-217).  Here each call validates its arguments,
+// allocates nothing, and enqueues its kernels on the caller's stream.
+
+#include <hip/hip_runtime_api.h>
+
+#include "fbct_internal.hpp"
+
+namespace {
+
+using fbct::fail;
+
+fbct_status check_common(fbct_plan *plan, const void *in, const void *out, int64_t n, const char *what)
+{
+    if (!plan) return fail(FBCT_ERR_INVALID_ARGUMENT, "%s: null plan", what);
+    if (n < 0) return fail(FBCT_ERR_INVALID_ARGUMENT, "%s: negative slice count %lld", what, (long long)n);
+    if (n > 0 && (!in || !out)) return fail(FBCT_ERR_INVALID_ARGUMENT, "%s: null data pointer", what);
+    if (n > 65535) return fail(FBCT_ERR_UNSUPPORTED, "%s: at most 65535 slices per call", what);
+    return FBCT_OK;
+}
+
+size_t sino_bytes(const fbct_plan *p, int64_t n)
+{
+    return static_cast<size_t>(n) * p->g.nViews * p->g.nCols * sizeof(float);
+}
+
+}  // namespace
+
+extern "C" {
+
+fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fp");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_fp(*plan, *t, image, sino, n_slices, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fp_adjoint");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_fp_adjoint(*plan, *t, sino, image, n_slices, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_bp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_bp");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_bp(*plan, *t, sino, image, n_slices, false, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, image, sino, n_slices, "fbct_bp_adjoint");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, false, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows, void *stream)
+{
+    if (!plan) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: null plan");
+    if (n_rows < 0) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: negative row count");
+    if (n_rows == 0) return FBCT_OK;
+    if (!in || !out) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: null data pointer");
+    if (in == out) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: in and out must not overlap");
+    const fbct::DeviceTables *t = nullptr;
+    fbct_status s;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_ramp_filter(*plan, *t, in, out, n_rows, static_cast<hipStream_t>(stream));
+}
+
+size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices)
+{
+    if (!plan || n_slices <= 0) return 0;
+    return sino_bytes(plan, n_slices);
+}
+
+fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
+        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp: workspace of %zu bytes needed, got %zu",
+                    sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float *q = static_cast<float *>(workspace);
+    if ((s = fbct::launch_ramp_filter(*plan, *t, sino, q, (int64_t)n_slices * plan->g.nViews, st)) != FBCT_OK)
+        return s;
+    return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);
+}
+
+fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices, void *workspace,
+                             size_t workspace_bytes, void *stream)
+{
+    fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fbp_adjoint");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
+        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_adjoint: workspace of %zu bytes needed, got %zu",
+                    sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float *q = static_cast<float *>(workspace);
+    if ((s = fbct::launch_bp_adjoint(*plan, *t, image, q, n_slices, true, st)) != FBCT_OK) return s;
+    // The centre-cropped convolution matrix H[c][m] = h[c-m] is symmetric (h[-n] = h[n]).
+    return fbct::launch_ramp_filter(*plan, *t, q, sino, (int64_t)n_slices * plan->g.nViews, st);
+}
+
+fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    // nZ = nRows = 1 (FDK/FBP.cc:85,98): the z-loop of FDK/GPURecon_GPU.cu:41-47 runs
+    // once at z = zmin + dz/2 = 0 and the single-row texture clamps the row index,
+    // so the result is the FBP of each slice with the same dShift and filter scale.
+    return fbct_fbp(plan, proj, vol, n_slices, workspace, workspace_bytes, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// fbct_internal.hpp -- internal types shared by the host library and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fbct.h"
+
+namespace fbct {
+
+// FP ray-march tiling (see fp.hip): rays per workgroup bundle and slab width in pixels.
+constexpr int kFpThreads = 256;
+constexpr int kFpLanesPerRay = 16;
+constexpr int kFpRays = kFpThreads / kFpLanesPerRay;
+constexpr int kFpSlab = 32;
+
+// FP adjoint output tile (see fp_adjoint in fp.hip).
+constexpr int kFpaTile = 32;
+
+// Device-resident copy of a plan's tables on one HIP device.
+struct DeviceTables {
+    int device = -1;
+    int32_t *ray_n = nullptr;      // [nViews*nCols]
+    float4 *ray_geo = nullptr;     // [nViews*nCols] (u0, w0, du, dw)
+    uint8_t *fp_major = nullptr;   // [nViews*nFpBundles] 1 = x-major
+    float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
+    double *ramp = nullptr;        // [2*nCols-1]
+    int32_t *fpa_cols = nullptr;   // [nTiles*nViews*2] candidate column range per tile/view
+};
+
+// Scalars the kernels need, derived once in FP64 on the host.
+struct Derived {
+    float fp_scale;       // 0.125 * dx
+    float two_r;          // 2 * rFOV/dx (FP sample coordinate extent)
+    int fp_nslab;         // slabs along the major axis
+    int fp_cap;           // LDS rows reserved for the minor extent of a slab
+    int fp_nbundles;      // ceil(nCols / kFpRays)
+    float bp_xi_scale;    // 1/dCol
+    float bp_xi_offset;   // dShift - dLeft/dCol
+    float r2;             // rFOV*rFOV (float, as the reference)
+    int fpa_tiles_x, fpa_tiles_y;
+};
+
+}  // namespace fbct
+
+struct fbct_plan {
+    fbct_fan_geom g{};
+    std::vector<float> angles;
+
+    // host tables (FP64 computed, stored at the precision the kernels read)
+    std::vector<int32_t> ray_n;
+    std::vector<float> ray_geo;     // 4 per ray
+    std::vector<uint8_t> fp_major;
+    std::vector<float> view_cs;     // 2 per view
+    std::vector<double> ramp;       // 2*nCols-1
+    std::vector<int32_t> fpa_cols;  // 2 per (tile, view)
+    int64_t fp_samples = 0;
+    fbct::Derived d{};
+
+    std::mutex mu;
+    std::vector<fbct::DeviceTables> dev;
+};
+
+namespace fbct {
+
+// Thread-local error reporting.
+fbct_status fail(fbct_status s, const char *fmt, ...);
+fbct_status ok();
+
+// Get (uploading if needed) the tables for the current device.
+fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
+
+// Kernel launchers (fp.hip, bp.hip, filter.hip).  All asynchronous on `stream`.
+fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const float *img, float *sino,
+                      int n, hipStream_t stream);
+fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *sino,
+                              float *img, int n, hipStream_t stream);
+fbct_status launch_bp(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img,
+                      int n, bool weighted, hipStream_t stream);
+fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *img,
+                              float *sino, int n, bool weighted, hipStream_t stream);
+fbct_status launch_ramp_filter(const fbct_plan &p, const DeviceTables &t, const float *in,
+                               float *out, int64_t rows, hipStream_t stream);
+
+}  // namespace fbct

@@ -1,0 +1,467 @@
+// plan.cpp -- parameter decode, FP64 geometry tables, plan lifecycle, error reporting.
+//
+// Everything here runs on the host.  Geometry that the kernels consume per ray or per
+// view is computed once in double precision and stored at the precision the kernels
+// read (FP32 positions, exact integer sample counts, FP64 ramp taps).
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "fbct_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// The reference's PI macro (FP/transform.h:1): float for the attribute decode, as a
+// double literal for the ramp taps (FBP/transform.cu:158).
+constexpr float kRefPiF = 3.14159265359f;
+constexpr double kRefPiD = 3.14159265359;
+
+}  // namespace
+
+namespace fbct {
+
+fbct_status fail(fbct_status s, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return s;
+}
+
+fbct_status ok()
+{
+    return FBCT_OK;
+}
+
+// ---------------------------------------------------------------- decode
+// FP/transform.cc:59-76 (identical in BP/FBP transform.cc and FDK/FBP.cc:61-84):
+// float arithmetic in the reference's operation order.
+static fbct_status decode(const float *p, fbct_fan_geom *g)
+{
+    std::memset(g, 0, sizeof(*g));
+    g->D = p[0];
+    g->R = p[1];
+    g->nViews = static_cast<int32_t>(p[2]);
+    g->dRange = p[3];
+    g->nCols = static_cast<int32_t>(p[4]);
+    g->nX = static_cast<int32_t>(p[5]);
+    g->nY = static_cast<int32_t>(p[6]);
+    g->rFOV = p[7];
+    g->dShift = p[9];
+    g->nSamp = static_cast<int32_t>(p[11]);
+    g->detBottom = p[13];
+    g->dRow = p[14];
+    g->dz = p[15];
+    if (g->nViews <= 0 || g->nCols <= 0 || g->nX <= 0 || g->nY <= 0)
+        return fail(FBCT_ERR_INVALID_GEOMETRY, "nViews, nCols, nX, nY must be positive (got %d, %d, %d, %d)",
+                    g->nViews, g->nCols, g->nX, g->nY);
+    volatile float t;
+    t = g->dRange / static_cast<float>(g->nViews);
+    t = t * kRefPiF;
+    g->dAngle = t / 180.0f;
+    t = 2.0f * g->rFOV;
+    g->dx = t / static_cast<float>(g->nX);
+    t = 2.0f * g->rFOV;
+    g->dy = t / static_cast<float>(g->nY);
+    t = -0.5f * static_cast<float>(g->nX);
+    g->xmin = t * g->dx;
+    t = -0.5f * static_cast<float>(g->nY);
+    g->ymin = t * g->dy;
+    t = p[8] * kRefPiF;
+    g->dCol = t / 180.0f;
+    t = p[10] * kRefPiF;
+    g->dLeft = t / 180.0f;
+    t = p[12] * kRefPiF;
+    g->startAngle = t / 180.0f;
+    return FBCT_OK;
+}
+
+static fbct_status validate(const fbct_fan_geom &g)
+{
+    if (!(g.rFOV > 0.0f) || !std::isfinite(g.rFOV))
+        return fail(FBCT_ERR_INVALID_GEOMETRY, "FOV radius must be positive (got %g)", (double)g.rFOV);
+    if (!(g.R > g.rFOV) || !std::isfinite(g.R))
+        return fail(FBCT_ERR_INVALID_GEOMETRY,
+                    "source-isocentre distance R=%g must exceed the FOV radius %g", (double)g.R,
+                    (double)g.rFOV);
+    if (!(g.dCol != 0.0f) || !std::isfinite(g.dCol) || !std::isfinite(g.dLeft) || !std::isfinite(g.dShift))
+        return fail(FBCT_ERR_INVALID_GEOMETRY, "column pitch must be finite and non-zero");
+    if ((int64_t)g.nX * g.nY > (int64_t(1) << 30) || (int64_t)g.nViews * g.nCols > (int64_t(1) << 30))
+        return fail(FBCT_ERR_INVALID_GEOMETRY, "image or sinogram larger than 2^30 elements");
+    if (g.nX > 16384 || g.nY > 16384 || g.nCols > 65536)
+        return fail(FBCT_ERR_INVALID_GEOMETRY, "image side or detector width too large");
+    return FBCT_OK;
+}
+
+// ------------------------------------------------------------ FP ray table
+// Per ray (view v, column c), FP/transform.cu:64-101 restated in FP64 in pixel units of
+// dx (the reference launches it with R/dx and rFOV/dx, :239).  Sample k of the ray is
+// at (u0 + k du, w0 + k dw) in texel-centre coordinates (the +0.5 texel offset of the
+// unnormalised linear texture is folded into u0, w0).
+struct RayGeo {
+    int n;
+    double u0, w0, du, dw;
+};
+
+static RayGeo ray_geometry(const fbct_fan_geom &g, double beta, int c)
+{
+    const double Rp = static_cast<double>(g.R) / static_cast<double>(g.dx);
+    const double r = static_cast<double>(g.rFOV) / static_cast<double>(g.dx);
+    const double gam = static_cast<double>(g.dLeft) + (static_cast<double>(c) + 0.5) * static_cast<double>(g.dCol);
+    const double th = beta + gam;
+    const double d = Rp * std::sin(gam);
+    const double disc = r * r - d * d;
+    RayGeo out{0, 0.0, 0.0, 0.0, 0.0};
+    // Missing the FOV circle gives NaN geometry and int(NaN) = 0 samples in the
+    // reference; a chord under one step gives floor(L/0.125) = 0.
+    if (!(disc > 0.0)) return out;
+    const double a = std::sqrt(disc);
+    const double nf = std::floor(16.0 * a);  // floor(L / 0.125), L = 2a
+    if (nf < 1.0) return out;
+    const int n = static_cast<int>(nf);
+    const double ct = std::cos(th), st = std::sin(th);
+    out.n = n;
+    out.du = (2.0 * a * ct) / static_cast<double>(n);
+    out.dw = (2.0 * a * st) / static_cast<double>(n);
+    out.u0 = (-a * ct + d * st) + r - 0.5;  // Rot_th(-a, -d) + r - 1/2
+    out.w0 = (-a * st - d * ct) + r - 0.5;
+    return out;
+}
+
+// Fan angle of a point (x, y) in mm as seen from the source of view (cb, sb):
+// BP/transform.cu:82-85 with the negated sin/cos passed at :116.
+static double fan_angle(const fbct_fan_geom &g, double cb, double sb, double x, double y)
+{
+    const double U = static_cast<double>(g.R) - x * cb - y * sb;
+    const double t = x * sb - y * cb;
+    return std::atan2(t, U);
+}
+
+static void build_tables(fbct_plan &p)
+{
+    const fbct_fan_geom &g = p.g;
+    const int nV = g.nViews, nC = g.nCols;
+    const double r = static_cast<double>(g.rFOV) / static_cast<double>(g.dx);
+    const double Rp = static_cast<double>(g.R) / static_cast<double>(g.dx);
+
+    p.ray_n.assign(static_cast<size_t>(nV) * nC, 0);
+    p.ray_geo.assign(static_cast<size_t>(nV) * nC * 4, 0.0f);
+    p.view_cs.assign(static_cast<size_t>(nV) * 2, 0.0f);
+    int64_t samples = 0;
+    for (int v = 0; v < nV; ++v) {
+        const double beta = static_cast<double>(p.angles[v]);
+        p.view_cs[2 * v] = static_cast<float>(std::cos(beta));
+        p.view_cs[2 * v + 1] = static_cast<float>(std::sin(beta));
+        for (int c = 0; c < nC; ++c) {
+            const RayGeo rg = ray_geometry(g, beta, c);
+            const size_t k = static_cast<size_t>(v) * nC + c;
+            p.ray_n[k] = rg.n;
+            p.ray_geo[4 * k + 0] = static_cast<float>(rg.u0);
+            p.ray_geo[4 * k + 1] = static_cast<float>(rg.w0);
+            p.ray_geo[4 * k + 2] = static_cast<float>(rg.du);
+            p.ray_geo[4 * k + 3] = static_cast<float>(rg.dw);
+            samples += rg.n;
+        }
+    }
+    p.fp_samples = samples;
+
+    // FP bundles: rays [b*kFpRays, (b+1)*kFpRays) of a view march slab by slab along
+    // the axis their central ray is closest to.  fp_cap bounds the minor extent of the
+    // taps of one slab, from the ray lines in FP64 plus a margin that covers the FP32
+    // sample positions; the kernel derives the exact extent from its own positions.
+    Derived &d = p.d;
+    d.fp_scale = static_cast<float>(0.125 * static_cast<double>(g.dx));
+    d.two_r = static_cast<float>(2.0 * r);
+    d.fp_nslab = static_cast<int>(std::floor((2.0 * r + 1.0) / kFpSlab)) + 1;
+    d.fp_nbundles = (nC + kFpRays - 1) / kFpRays;
+    p.fp_major.assign(static_cast<size_t>(nV) * d.fp_nbundles, 1);
+    double cap = 0.0;
+    for (int v = 0; v < nV; ++v) {
+        const double beta = static_cast<double>(p.angles[v]);
+        const double su = Rp * std::cos(beta) + r - 0.5;  // source, texel coordinates
+        const double sw = Rp * std::sin(beta) + r - 0.5;
+        for (int b = 0; b < d.fp_nbundles; ++b) {
+            const int c0 = b * kFpRays;
+            const int c1 = std::min(nC, c0 + kFpRays) - 1;
+            const double gmid = static_cast<double>(g.dLeft) + (0.5 * (c0 + c1) + 0.5) * static_cast<double>(g.dCol);
+            const double thm = beta + gmid;
+            const bool xmajor = std::fabs(std::cos(thm)) >= std::fabs(std::sin(thm));
+            p.fp_major[static_cast<size_t>(v) * d.fp_nbundles + b] = xmajor ? 1 : 0;
+            const double sa = xmajor ? su : sw, sm = xmajor ? sw : su;
+            double slope_max = 0.0, lo_a = 1e300, hi_a = -1e300, lo_b = 1e300, hi_b = -1e300;
+            for (int c : {c0, c1}) {
+                const double th = beta + static_cast<double>(g.dLeft) + (c + 0.5) * static_cast<double>(g.dCol);
+                const double da = xmajor ? std::cos(th) : std::sin(th);
+                const double dm = xmajor ? std::sin(th) : std::cos(th);
+                const double slope = dm / da;
+                slope_max = std::max(slope_max, std::fabs(slope));
+                const double ma = sm + (-1.0 - sa) * slope;
+                const double mb = sm + (2.0 * r + 1.0 - sa) * slope;
+                lo_a = std::min(lo_a, ma);
+                hi_a = std::max(hi_a, ma);
+                lo_b = std::min(lo_b, mb);
+                hi_b = std::max(hi_b, mb);
+            }
+            const double spread = std::max(hi_a - lo_a, hi_b - lo_b);
+            cap = std::max(cap, spread + kFpSlab * slope_max + 4.0);
+        }
+    }
+    d.fp_cap = static_cast<int>(std::ceil(std::min(cap, std::ceil(2.0 * r) + 3.0)));
+
+    // BP: xi_raw = (atan(t/U) - dLeft)/dCol + dShift  (BP/transform.cu:85).
+    d.bp_xi_scale = static_cast<float>(1.0 / static_cast<double>(g.dCol));
+    d.bp_xi_offset = static_cast<float>(static_cast<double>(g.dShift) -
+                                        static_cast<double>(g.dLeft) / static_cast<double>(g.dCol));
+    d.r2 = g.rFOV * g.rFOV;
+
+    // Ramp taps, FBP/transform.cu:153-162 (Hsieh eq. 3.51, scaled by R*dAngle*dCol).
+    p.ramp.assign(static_cast<size_t>(2 * nC - 1), 0.0);
+    const double dc = static_cast<double>(g.dCol);
+    const double kappa = static_cast<double>(g.R) * static_cast<double>(g.dAngle) * dc;
+    for (int n = -(nC - 1); n <= nC - 1; ++n) {
+        double h;
+        if (n == 0) {
+            h = 0.5 / (4.0 * dc * dc);
+        } else if (n % 2 != 0) {
+            const double s = std::sin(static_cast<double>(n) * dc);
+            h = -0.5 / (kRefPiD * kRefPiD * s * s);
+        } else {
+            h = 0.0;
+        }
+        p.ramp[n + nC - 1] = h * kappa;
+    }
+
+    // FP adjoint: for each output tile and view, the columns whose rays can touch the
+    // tile's taps (texel range [t0-1, t0+T] per axis), from the fan angles of the tile
+    // corners (convex region; the source lies outside it), with one column of margin.
+    d.fpa_tiles_x = (g.nX + kFpaTile - 1) / kFpaTile;
+    d.fpa_tiles_y = (g.nY + kFpaTile - 1) / kFpaTile;
+    const int ntiles = d.fpa_tiles_x * d.fpa_tiles_y;
+    p.fpa_cols.assign(static_cast<size_t>(ntiles) * nV * 2, 0);
+    const double dxd = static_cast<double>(g.dx);
+    for (int ty = 0; ty < d.fpa_tiles_y; ++ty)
+        for (int tx = 0; tx < d.fpa_tiles_x; ++tx) {
+            const double ua = tx * kFpaTile - 1.0, ub = std::min(g.nX, (tx + 1) * kFpaTile) + 0.0;
+            const double wa = ty * kFpaTile - 1.0, wb = std::min(g.nY, (ty + 1) * kFpaTile) + 0.0;
+            // texel-centre coordinate u maps to x = (u + 0.5 - r) dx (FP uses dx for both axes)
+            const double xs[2] = {(ua + 0.5 - r) * dxd, (ub + 0.5 - r) * dxd};
+            const double ys[2] = {(wa + 0.5 - r) * dxd, (wb + 0.5 - r) * dxd};
+            const double corner_r = std::max(std::hypot(xs[0], ys[0]), std::max(std::hypot(xs[1], ys[0]),
+                                    std::max(std::hypot(xs[0], ys[1]), std::hypot(xs[1], ys[1]))));
+            const bool convex_ok = corner_r < 0.999 * static_cast<double>(g.R);
+            const int tile = ty * d.fpa_tiles_x + tx;
+            for (int v = 0; v < nV; ++v) {
+                int32_t *cr = &p.fpa_cols[(static_cast<size_t>(tile) * nV + v) * 2];
+                if (!convex_ok) {
+                    cr[0] = 0;
+                    cr[1] = nC - 1;
+                    continue;
+                }
+                const double beta = static_cast<double>(p.angles[v]);
+                const double cb = std::cos(beta), sb = std::sin(beta);
+                double gmin = 1e300, gmax = -1e300;
+                for (double x : xs)
+                    for (double y : ys) {
+                        const double ga = fan_angle(g, cb, sb, x, y);
+                        gmin = std::min(gmin, ga);
+                        gmax = std::max(gmax, ga);
+                    }
+                double ca = (gmin - static_cast<double>(g.dLeft)) / dc - 0.5;
+                double cb2 = (gmax - static_cast<double>(g.dLeft)) / dc - 0.5;
+                if (ca > cb2) std::swap(ca, cb2);
+                const double lo = std::floor(ca) - 1.0, hi = std::ceil(cb2) + 1.0;
+                cr[0] = static_cast<int32_t>(std::max(0.0, std::min(lo, nC - 1.0)));
+                cr[1] = static_cast<int32_t>(std::max(-1.0, std::min(hi, nC - 1.0)));
+                if (hi < 0.0 || lo > nC - 1.0) {
+                    cr[0] = 0;
+                    cr[1] = -1;  // empty
+                }
+            }
+        }
+}
+
+template <typename T>
+static fbct_status upload(T **dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) bytes = sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), bytes);
+    if (e != hipSuccess) return fail(FBCT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    if (src) {
+        e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    return FBCT_OK;
+}
+
+static void free_tables(DeviceTables &t)
+{
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) cur = -1;
+    if (t.device >= 0) (void)hipSetDevice(t.device);
+    (void)hipFree(t.ray_n);
+    (void)hipFree(t.ray_geo);
+    (void)hipFree(t.fp_major);
+    (void)hipFree(t.view_cs);
+    (void)hipFree(t.ramp);
+    (void)hipFree(t.fpa_cols);
+    if (cur >= 0) (void)hipSetDevice(cur);
+    t = DeviceTables{};
+}
+
+fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(plan->mu);
+    for (const DeviceTables &t : plan->dev)
+        if (t.device == dev) {
+            *out = &t;
+            return FBCT_OK;
+        }
+    DeviceTables t;
+    t.device = dev;
+    fbct_status s;
+    if ((s = upload(&t.ray_n, plan->ray_n.data(), plan->ray_n.size() * sizeof(int32_t))) != FBCT_OK ||
+        (s = upload(&t.ray_geo, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float))) != FBCT_OK ||
+        (s = upload(&t.fp_major, plan->fp_major.data(), plan->fp_major.size())) != FBCT_OK ||
+        (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
+        (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK ||
+        (s = upload(&t.fpa_cols, plan->fpa_cols.data(), plan->fpa_cols.size() * sizeof(int32_t))) != FBCT_OK) {
+        std::string msg = g_last_error;
+        free_tables(t);
+        g_last_error = msg;
+        return s;
+    }
+    // Plans are tiny; keep the vector from reallocating so returned pointers stay valid.
+    if (plan->dev.capacity() == 0) plan->dev.reserve(16);
+    if (plan->dev.size() == plan->dev.capacity())
+        return fail(FBCT_ERR_UNSUPPORTED, "plan used on more than %zu devices", plan->dev.capacity());
+    plan->dev.push_back(t);
+    *out = &plan->dev.back();
+    return FBCT_OK;
+}
+
+}  // namespace fbct
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+int fbct_version(void)
+{
+    return FBCT_VERSION;
+}
+
+const char *fbct_status_string(fbct_status s)
+{
+    switch (s) {
+    case FBCT_OK: return "ok";
+    case FBCT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case FBCT_ERR_INVALID_GEOMETRY: return "invalid geometry";
+    case FBCT_ERR_SHAPE_MISMATCH: return "shape mismatch";
+    case FBCT_ERR_UNSUPPORTED: return "unsupported";
+    case FBCT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case FBCT_ERR_DEVICE: return "device error";
+    case FBCT_ERR_WORKSPACE: return "workspace missing or too small";
+    }
+    return "unknown status";
+}
+
+const char *fbct_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+fbct_status fbct_geom_from_params(const float *parameter, int n_parameter, fbct_fan_geom *out)
+{
+    if (!parameter || !out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null pointer");
+    if (n_parameter < 16)
+        return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "parameter needs 16 values, got %d", n_parameter);
+    fbct_status s = fbct::decode(parameter, out);
+    if (s != FBCT_OK) return s;
+    return fbct::validate(*out);
+}
+
+fbct_status fbct_plan_create(const float *parameter, int n_parameter, const float *angles, int n_angles,
+                             fbct_plan **out)
+{
+    if (!out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null plan output pointer");
+    *out = nullptr;
+    fbct_fan_geom g;
+    fbct_status s = fbct_geom_from_params(parameter, n_parameter, &g);
+    if (s != FBCT_OK) return s;
+    if (!angles) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null angle list");
+    if (n_angles != g.nViews)
+        return fbct::fail(FBCT_ERR_SHAPE_MISMATCH, "angle list has %d entries but parameter[2] gives %d views",
+                          n_angles, g.nViews);
+    for (int v = 0; v < n_angles; ++v)
+        if (!std::isfinite(angles[v])) return fbct::fail(FBCT_ERR_INVALID_GEOMETRY, "angle %d is not finite", v);
+    fbct_plan *p = new (std::nothrow) fbct_plan();
+    if (!p) return fbct::fail(FBCT_ERR_OUT_OF_MEMORY, "plan allocation");
+    try {
+        p->g = g;
+        p->angles.assign(angles, angles + n_angles);
+        fbct::build_tables(*p);
+    } catch (const std::bad_alloc &) {
+        delete p;
+        return fbct::fail(FBCT_ERR_OUT_OF_MEMORY, "plan tables");
+    }
+    *out = p;
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_destroy(fbct_plan *plan)
+{
+    if (!plan) return FBCT_OK;
+    for (fbct::DeviceTables &t : plan->dev) fbct::free_tables(t);
+    delete plan;
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_geom(const fbct_plan *plan, fbct_fan_geom *out)
+{
+    if (!plan || !out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null pointer");
+    *out = plan->g;
+    return FBCT_OK;
+}
+
+int64_t fbct_plan_fp_samples(const fbct_plan *plan)
+{
+    return plan ? plan->fp_samples : -1;
+}
+
+fbct_status fbct_plan_ray_table(const fbct_plan *plan, int32_t *n_out, float *geo_out)
+{
+    if (!plan) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null plan");
+    if (n_out) std::memcpy(n_out, plan->ray_n.data(), plan->ray_n.size() * sizeof(int32_t));
+    if (geo_out) std::memcpy(geo_out, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float));
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_ramp_taps(const fbct_plan *plan, double *taps_out)
+{
+    if (!plan || !taps_out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null pointer");
+    std::memcpy(taps_out, plan->ramp.data(), plan->ramp.size() * sizeof(double));
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_upload(fbct_plan *plan)
+{
+    if (!plan) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null plan");
+    const fbct::DeviceTables *t = nullptr;
+    return fbct::device_tables(plan, &t);
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""Drop-in operators with the reference op signatures, differentiable through
+hand-written matched adjoints.
+
+Reference ops (TF1 custom ops, GPU kernels only):
+
+=====================  ===================================  ==========================
+python name            registration                         here
+=====================  ===================================  ==========================
+``fp_fan_curve``       ``FPFanCurve``  FP/transform.cc:17   :func:`fp_fan_curve`
+``bp_fan_curve``       ``BPFanCurve``  BP/transform.cc:17   :func:`bp_fan_curve`
+``fbp_fan_curve``      ``FBPFanCurve`` FBP/transform.cc:17  :func:`fbp_fan_curve`
+``BackProjection``     ``BackProjection`` FDK/FBP.cc:16     :func:`BackProjection`
+=====================  ===================================  ==========================
+
+Each takes ``(volume|projection, angle, parameter, vol_shape, proj_shape)`` exactly as
+the reference is called (utils.py:25,35,45; FDK/FDK_test_training.py:111) and returns a
+tensor of the attribute-declared output shape (FP/transform.cc:29-33).  Unlike the
+reference, which trusts ``parameter[11]`` for the slice count (FP/transform.cc:75), the
+input's element count is checked against it.
+
+Gradients: FP's backward is the exact transpose of its interpolation weights
+(``fbct_fp_adjoint``); FBP's backward is filter^T o weighted-BP^T (``fbct_fbp_adjoint``);
+BP's backward is BP^T (``fbct_bp_adjoint``).  The operators are linear, so nothing from
+the forward pass is kept for the backward pass.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from .plan import FanBeamPlan, get_plan
+
+__all__ = [
+    "fp_fan_curve", "bp_fan_curve", "fbp_fan_curve", "BackProjection",
+    "forward_project", "back_project", "filtered_back_project", "fdk",
+    "FPFunction", "BPFunction", "FBPFunction",
+]
+
+
+class FPFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, image, plan: FanBeamPlan):
+        ctx.plan = plan
+        return plan.fp(image)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.plan.fp_adjoint(grad.contiguous()), None
+
+
+class FPAdjointFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sino, plan: FanBeamPlan):
+        ctx.plan = plan
+        return plan.fp_adjoint(sino)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.plan.fp(grad.contiguous()), None
+
+
+class BPFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sino, plan: FanBeamPlan):
+        ctx.plan = plan
+        return plan.bp(sino)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.plan.bp_adjoint(grad.contiguous()), None
+
+
+class FBPFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sino, plan: FanBeamPlan):
+        ctx.plan = plan
+        return plan.fbp(sino)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return ctx.plan.fbp_adjoint(grad.contiguous()), None
+
+
+def forward_project(image: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
+    """Differentiable FP: image [n, nY, nX] -> sinogram [n, nViews, nCols]."""
+    return FPFunction.apply(image, plan)
+
+
+def fp_adjoint(sino: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
+    """Differentiable exact transpose of :func:`forward_project`."""
+    return FPAdjointFunction.apply(sino, plan)
+
+
+def back_project(sino: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
+    return BPFunction.apply(sino, plan)
+
+
+def filtered_back_project(sino: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
+    return FBPFunction.apply(sino, plan)
+
+
+def fdk(proj: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
+    """Reference FDK op semantics (nZ = nRows = 1): the FBP of each slice."""
+    return FBPFunction.apply(proj, plan)
+
+
+# ------------------------------------------------------------------ drop-ins
+
+def _slices(x: torch.Tensor, plan: FanBeamPlan, per_slice: int, what: str) -> torch.Tensor:
+    n = plan.n_samp
+    if n <= 0:
+        raise ValueError("%s: parameter[11] (slice count) must be positive, got %d" % (what, n))
+    if x.numel() != n * per_slice:
+        raise ValueError("%s: input has %d elements but parameter[11]=%d slices need %d"
+                         % (what, x.numel(), n, n * per_slice))
+    return x
+
+
+def _out(y: torch.Tensor, shape, what: str) -> torch.Tensor:
+    shape = tuple(int(s) for s in shape)
+    if math.prod(shape) != y.numel():
+        raise ValueError("%s: declared output shape %s holds %d elements, result has %d"
+                         % (what, shape, math.prod(shape), y.numel()))
+    return y.reshape(shape)
+
+
+def fp_fan_curve(volume, angle, parameter, vol_shape, proj_shape):
+    """``FPFanCurve`` (FP/transform.cc:17-37,93-117): volume -> projection."""
+    plan = get_plan(parameter, angle)
+    x = _slices(volume, plan, plan.nx * plan.ny, "fp_fan_curve").reshape(plan.n_samp, plan.ny, plan.nx)
+    return _out(forward_project(x, plan), proj_shape, "fp_fan_curve")
+
+
+def bp_fan_curve(projection, angle, parameter, vol_shape, proj_shape):
+    """``BPFanCurve`` (BP/transform.cc:17-36,92-116): projection -> volume (unweighted BP)."""
+    plan = get_plan(parameter, angle)
+    p = _slices(projection, plan, plan.n_views * plan.n_cols, "bp_fan_curve")
+    p = p.reshape(plan.n_samp, plan.n_views, plan.n_cols)
+    return _out(back_project(p, plan), vol_shape, "bp_fan_curve")
+
+
+def fbp_fan_curve(projection, angle, parameter, vol_shape, proj_shape):
+    """``FBPFanCurve`` (FBP/transform.cc:17-36,92-116): ramp filter + 1/L^2-weighted BP."""
+    plan = get_plan(parameter, angle)
+    p = _slices(projection, plan, plan.n_views * plan.n_cols, "fbp_fan_curve")
+    p = p.reshape(plan.n_samp, plan.n_views, plan.n_cols)
+    return _out(filtered_back_project(p, plan), vol_shape, "fbp_fan_curve")
+
+
+def BackProjection(projection, angle, parameter, vol_shape, proj_shape):  # noqa: N802 (reference name)
+    """The FDK op ``BackProjection`` (FDK/FBP.cc:16-35,125-152).
+
+    The reference decodes nZ = nRows = 1 (FDK/FBP.cc:85,98), ignores parameter[11] and
+    processes the first slice only (FDK/GPURecon_GPU.cu:178-179).  Here the slice count
+    is taken from the input (all slices are reconstructed); with one slice the result
+    is the reference's.
+    """
+    plan = get_plan(parameter, angle)
+    per = plan.n_views * plan.n_cols
+    if projection.numel() % per != 0:
+        raise ValueError("BackProjection: input size %d is not a multiple of nViews*nCols=%d"
+                         % (projection.numel(), per))
+    p = projection.reshape(projection.numel() // per, plan.n_views, plan.n_cols)
+    return _out(fdk(p, plan), vol_shape, "BackProjection")

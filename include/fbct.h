@@ -1,0 +1,171 @@
+/*
+ * fbct.h -- C ABI of libfbct, the MI355X (gfx950) fan-beam CT operator library.
+ *
+ * Drop-in replacement for the four TF1 custom ops of xtie97/FanBeamCTRecon_TensorflowOp
+ * and the host entry points behind them:
+ *
+ *   reference op (python name)           reference host entry point              here
+ *   FPFanCurve  (fp_fan_curve)           project_fan_curve     FP/transform.h:3    fbct_fp
+ *   BPFanCurve  (bp_fan_curve)           backproject_fan_curve BP/transform.h:3    fbct_bp
+ *   FBPFanCurve (fbp_fan_curve)          backproject_fan_curve FBP/transform.h:3   fbct_fbp
+ *   BackProjection (FDK op)              BP_One_Rot_3D_GPU FDK/GPURecon_GPU.h:59   fbct_fdk
+ *   attribute decode (op constructors)   para_setup_fan_curve  FP/transform.h:2    fbct_geom_from_params,
+ *                                                                                  fbct_plan_create
+ *
+ * Conventions (identical to the reference memory layouts, FP/transform.cu:164,250 and
+ * BP/transform.cu:87):
+ *   image     float[n_slices][nY][nX]           (i fastest)
+ *   sinogram  float[n_slices][nViews][nCols]    (column fastest)
+ * All data pointers are DEVICE pointers (HIP).  `stream` is a hipStream_t passed as
+ * void* (NULL = the default stream).  Every execute call is asynchronous and
+ * stream-ordered; none synchronises the device or allocates memory, so a call
+ * sequence can be captured into a HIP graph once the plan's tables are resident on
+ * the device (fbct_plan_upload, or any earlier execute on that device).
+ *
+ * Errors are returned as fbct_status; nothing ever exits the process (the reference
+ * exits through checkCudaErrors, FP/transform.cu:4).  fbct_last_error() gives a
+ * thread-local message for the last failing call on the calling thread.
+ */
+#ifndef FBCT_H
+#define FBCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBCT_VERSION 100 /* 1.0.0 */
+
+typedef enum fbct_status {
+    FBCT_OK = 0,
+    FBCT_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative count, bad enum       */
+    FBCT_ERR_INVALID_GEOMETRY = 2, /* parameter vector describes no valid scanner  */
+    FBCT_ERR_SHAPE_MISMATCH = 3,   /* slice count / angle count inconsistent       */
+    FBCT_ERR_UNSUPPORTED = 4,      /* valid request this build does not implement  */
+    FBCT_ERR_OUT_OF_MEMORY = 5,
+    FBCT_ERR_DEVICE = 6,           /* a HIP runtime call failed                    */
+    FBCT_ERR_WORKSPACE = 7         /* workspace missing or too small               */
+} fbct_status;
+
+/* Storage type of images / sinograms in device memory.  Arithmetic is FP32
+ * (accumulation along each ray / over views in FP32; ramp filter in FP64). */
+typedef enum fbct_dtype {
+    FBCT_F32 = 0,
+    FBCT_BF16 = 1
+} fbct_dtype;
+
+/* Geometry decoded from the reference's 16-float `parameter` attribute exactly as
+ * the op constructors decode it (FP/transform.cc:59-76, FDK/FBP.cc:61-86):
+ * float arithmetic, PI = 3.14159265359f, angles converted from degrees. */
+typedef struct fbct_fan_geom {
+    float D;          /* [0]  source-detector distance, mm (FDK row map only) */
+    float R;          /* [1]  source-isocentre distance, mm                    */
+    int32_t nViews;   /* [2]                                                    */
+    float dRange;     /* [3]  angular range, deg                                */
+    float dAngle;     /*      dRange/nViews*PI/180, rad (filter scale)          */
+    int32_t nCols;    /* [4]                                                    */
+    int32_t nX, nY;   /* [5],[6]                                                */
+    float rFOV;       /* [7]  mm                                                */
+    float dx, dy;     /*      2*rFOV/nX, 2*rFOV/nY, mm                          */
+    float xmin, ymin; /*      -nX*dx/2, -nY*dy/2, mm                            */
+    float dCol;       /* [8]  rad                                               */
+    float dShift;     /* [9]  columns                                           */
+    float dLeft;      /* [10] rad                                               */
+    int32_t nSamp;    /* [11] slices                                            */
+    float startAngle; /* [12] rad (parsed, unused -- as in the reference)       */
+    float detBottom;  /* [13] mm (FDK)                                          */
+    float dRow;       /* [14] mm (FDK)                                          */
+    float dz;         /* [15] mm (FDK)                                          */
+} fbct_fan_geom;
+
+typedef struct fbct_plan fbct_plan;
+
+/* ---------------------------------------------------------------- host only --
+ * These never touch the GPU. */
+
+int fbct_version(void);
+const char *fbct_status_string(fbct_status s);
+const char *fbct_last_error(void);
+
+/* Decode the 16-float parameter attribute.  Replaces the attribute decoding in the op
+ * constructors FPFanCurveOp/BPFanCurveOp/FBPFanCurveOp (FP/transform.cc:50-91,
+ * BP/transform.cc:49-90, FBP/transform.cc:49-90) and BackProjectionOp (FDK/FBP.cc:53-123). */
+fbct_status fbct_geom_from_params(const float *parameter, int n_parameter, fbct_fan_geom *out);
+
+/* Create an immutable plan from the op attributes `parameter` (16 floats) and `angle`
+ * (nViews floats, rad).  Replaces para_setup_fan_curve (FP/transform.cu:20-62,
+ * BP/transform.cu:20-64, FBP/transform.cu:23-58), which kept the geometry in
+ * process-global state; a plan owns its geometry, so any number may coexist.
+ * Per-ray / per-view tables are computed here in FP64 on the host. */
+fbct_status fbct_plan_create(const float *parameter, int n_parameter, const float *angles,
+                             int n_angles, fbct_plan **out);
+fbct_status fbct_plan_destroy(fbct_plan *plan);
+fbct_status fbct_plan_geom(const fbct_plan *plan, fbct_fan_geom *out);
+
+/* FP samples of one slice: sum over rays of N = floor(chord/0.125 px). */
+int64_t fbct_plan_fp_samples(const fbct_plan *plan);
+
+/* Host copy of the FP ray table: n_out[nViews*nCols] sample counts and
+ * geo_out[nViews*nCols*4] = (u0, w0, du, dw) with u0,w0 the texel-centre
+ * coordinates of sample 0 (the FP32 values the kernels use).  Either may be NULL. */
+fbct_status fbct_plan_ray_table(const fbct_plan *plan, int32_t *n_out, float *geo_out);
+
+/* Host copy of the ramp taps h[n], n in [-(nCols-1), nCols-1] (2*nCols-1 doubles). */
+fbct_status fbct_plan_ramp_taps(const fbct_plan *plan, double *taps_out);
+
+/* ------------------------------------------------------------------- device --
+ * Upload the plan's tables to the current HIP device (done implicitly by the first
+ * execute on a device; call it before capturing a HIP graph). */
+fbct_status fbct_plan_upload(fbct_plan *plan);
+
+/* Fan-beam forward projection.  image[n][nY][nX] -> sino[n][nViews][nCols].
+ * Replaces project_fan_curve (FP/transform.cu:173-269; kernels :64-146). */
+fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slices, void *stream);
+
+/* Exact matched adjoint of fbct_fp (transpose of the same interpolation weights).
+ * sino -> image.  No reference counterpart (the reference has no gradient). */
+fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, int n_slices,
+                            void *stream);
+
+/* Pixel-driven backprojection (unweighted).  sino -> image.
+ * Replaces backproject_fan_curve of BP (BP/transform.cu:123-176; kernel :67-95). */
+fbct_status fbct_bp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream);
+
+/* Transpose of fbct_bp.  image -> sino.  No reference counterpart. */
+fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
+                            void *stream);
+
+/* Ramp filter of every row: out[r][c] = sum_m in[r][m] h[c-m] (centre-cropped full
+ * linear convolution; FP64 accumulation).  Replaces the host IPP filter
+ * (FBP/transform.cu:147-176, FDK/GPURecon_GPU.cu:184-217).  in and out must not
+ * overlap. */
+fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows,
+                             void *stream);
+
+/* Workspace in bytes needed by fbct_fbp / fbct_fbp_adjoint / fbct_fdk for n_slices. */
+size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices);
+
+/* Filtered backprojection: ramp filter + 1/L^2-weighted pixel-driven BP.
+ * sino -> image.  Replaces backproject_fan_curve of FBP (FBP/transform.cu:117-201;
+ * kernel :61-89).  workspace: device buffer of fbct_fbp_workspace_size bytes. */
+fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_slices,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Exact adjoint of fbct_fbp.  image -> sino.  No reference counterpart. */
+fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
+                             void *workspace, size_t workspace_bytes, void *stream);
+
+/* The reference FDK op ("BackProjection", FDK/FBP.cc:16-155 -> BP_One_Rot_3D_GPU,
+ * FDK/GPURecon_GPU.cu:163-268; kernel :15-50).  The reference pins nZ = nRows = 1
+ * (FDK/FBP.cc:85,98), which makes it the FBP of each slice; unlike the reference
+ * (slice 0 only, accumulator never zeroed, :178-179,236-238) every slice is
+ * processed and the output is fully written. */
+fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FBCT_H */

@@ -1,0 +1,142 @@
+"""CPU tests of the host side of libfbct: the library loads and exports every symbol the
+public header declares, validates its arguments without touching a GPU, and builds the
+same FP64 geometry tables as the oracle; plus the caller compatibility layer."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+from fanbeamctrecon_tensorflowop_amd import _lib, compat, workloads
+from fanbeamctrecon_tensorflowop_amd.plan import FanBeamPlan
+
+
+def header_functions():
+    src = open(os.path.join(REPO, "include", "fbct.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(fbct_[a-z_0-9]+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    # the ctypes signatures cover exactly the header
+    assert sorted(n for n, _, _ in _lib.SIGNATURES) == names
+
+
+def test_version_and_status_strings():
+    lib = _lib.load()
+    assert lib.fbct_version() == 100
+    assert lib.fbct_status_string(0) == b"ok"
+    assert lib.fbct_status_string(7).startswith(b"workspace")
+
+
+def _geom(p, n=16):
+    g = _lib.FanGeom()
+    st = _lib.load().fbct_geom_from_params(p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, ctypes.byref(g))
+    return st, g
+
+
+def test_geom_decode_matches_oracle(oracle_mod):
+    for p, a in (compat.param_set(3), compat.angle(984, 0)), workloads.clinical_scan(), workloads.small_scan():
+        st, g = _geom(p)
+        assert st == 0
+        o = oracle_mod.Oracle(p, a)
+        for name, _ in _lib.FanGeom._fields_:
+            assert getattr(g, name) == getattr(o.g, name), name
+
+
+@pytest.mark.parametrize("idx,val,status", [
+    (2, 0, 2),        # no views
+    (4, -3, 2),       # negative columns
+    (7, 0.0, 2),      # zero FOV
+    (1, 100.0, 2),    # source inside the FOV (R < rFOV)
+    (8, 0.0, 2),      # zero column pitch
+])
+def test_geom_validation(idx, val, status):
+    p = compat.param_set(1).copy()
+    p[idx] = val
+    st, _ = _geom(p)
+    assert st == status
+    assert _lib.last_error()
+
+
+def test_short_parameter_vector_rejected():
+    st, _ = _geom(compat.param_set(1), n=12)
+    assert st == 1
+
+
+def test_plan_rejects_angle_count_mismatch():
+    p = compat.param_set(1)
+    with pytest.raises(_lib.FbctError) as e:
+        FanBeamPlan(p, compat.angle(983, 0))
+    assert e.value.status == 3
+
+
+def test_execute_validation_without_gpu():
+    lib = _lib.load()
+    plan = FanBeamPlan(*workloads.tiny_scan())
+    h = plan._h
+    null = ctypes.c_void_p(0)
+    # zero slices is a no-op that never touches the device
+    assert lib.fbct_fp(h, null, null, 0, null) == 0
+    assert lib.fbct_bp(h, null, null, 0, null) == 0
+    # null data with slices, negative counts, missing workspace
+    assert lib.fbct_fp(h, null, null, 1, null) == 1
+    assert lib.fbct_fp_adjoint(h, null, null, -1, null) == 1
+    assert lib.fbct_fp(None, null, null, 1, null) == 1
+    fake = ctypes.c_void_p(0x1000)
+    assert lib.fbct_fbp(h, fake, fake, 1, null, 0, null) == 7
+    assert lib.fbct_fbp_workspace_size(h, 2) == 2 * plan.n_views * plan.n_cols * 4
+    assert lib.fbct_ramp_filter(h, fake, fake, 3, null) == 1  # in == out
+
+
+def test_ray_table_matches_oracle(oracle_mod):
+    for p, a in (workloads.small_scan(), workloads.clinical_scan()):
+        plan = FanBeamPlan(p, a)
+        o = oracle_mod.Oracle(p, a)
+        n, geo = plan.ray_table()
+        assert n.sum() == o.sample_count() == plan.fp_samples
+        rng = np.random.default_rng(1)
+        for _ in range(300):
+            v, c = int(rng.integers(0, len(a))), int(rng.integers(0, plan.n_cols))
+            N, vals = o.ray(v, c)
+            assert n[v, c] == N
+            if N:
+                np.testing.assert_array_equal(geo[v, c], np.asarray(vals, dtype=np.float32))
+
+
+def test_ramp_taps_match_oracle(oracle_mod):
+    p, a = workloads.clinical_scan()
+    np.testing.assert_allclose(FanBeamPlan(p, a).ramp_taps(), oracle_mod.Oracle(p, a).ramp_taps(), rtol=1e-15,
+                               atol=0)
+
+
+def test_compat_param_set_and_angle():
+    p = compat.param_set(2)
+    expect = np.array([946.73233346, 538.52, 984, 360, 888, 512, 512, 200, 55.0059 / 888, 0, -55.0059 / 2, 2, 0,
+                       13, 14, 15], dtype=np.float32)
+    np.testing.assert_array_equal(p, expect)
+    a = compat.angle(984, 0)
+    np.testing.assert_array_equal(a, (np.arange(984) * (2 * math.pi / 984)).astype(np.float32))
+    # the reference adds the start angle (degrees) to radians (utils.py:53)
+    np.testing.assert_array_equal(compat.angle(4, 10.0), (10.0 + np.arange(4) * (math.pi / 2)).astype(np.float32))
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libfbct.so"))
+    with pytest.raises(_lib.FbctLibraryMissing):
+        _lib.load()
+
+
+def test_tensor_checks_reject_cpu_tensors():
+    import torch
+    plan = FanBeamPlan(*workloads.tiny_scan())
+    with pytest.raises(ValueError, match="no CPU path"):
+        plan.fp(torch.zeros(1, 16, 16))
