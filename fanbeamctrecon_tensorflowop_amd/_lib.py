@@ -73,6 +73,8 @@ SIGNATURES = [
     ("fbct_fp_adjoint", _I, [_P, _P, _P, _I, _P]),
     ("fbct_bp", _I, [_P, _P, _P, _I, _P]),
     ("fbct_bp_adjoint", _I, [_P, _P, _P, _I, _P]),
+    ("fbct_fbp_backproject", _I, [_P, _P, _P, _I, _P]),
+    ("fbct_fbp_backproject_adjoint", _I, [_P, _P, _P, _I, _P]),
     ("fbct_ramp_filter", _I, [_P, _P, _P, ctypes.c_int64, _P]),
     ("fbct_fbp_workspace_size", ctypes.c_size_t, [_P, _I]),
     ("fbct_fbp", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),

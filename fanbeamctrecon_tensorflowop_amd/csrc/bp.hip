@@ -9,6 +9,11 @@
 // views with the per-view (cos, sin) read as uniform scalars, accumulates in a
 // register and stores once.  The detector row is read with two gathers per update;
 // the 1-D texture's clamp-to-edge linear filter is reproduced with exact FP32 weights.
+//
+// The reference's hard cut-off 0 < xi < nCols is a discontinuity: where a pixel sits
+// within 0.01 column of a fan edge the decision is retaken in FP64 (tangent test on
+// U, t from FP64 pixel coordinates and FP64 cos/sin), so FP32 rounding of xi cannot
+// flip it.
 
 #include <hip/hip_runtime.h>
 
@@ -21,35 +26,59 @@ struct BpArgs {
     const float *sino;
     float *img;
     const float2 *cs;
+    const double2 *csd;
     long long img_stride, sino_stride;
     int nX, nY, nViews, nCols;
     float R, xmin, ymin, dx, dy, r2;
     float xi_scale, xi_offset;
+    int edge_exact;
+    double tan_min, tan_max;
 };
 
 constexpr int kBpTx = 16, kBpTy = 16;
+constexpr float kEdgeBand = 0.01f;  // columns
 
-// Column coordinate of the pixel in view (cb, sb) and, optionally, its 1/L^2 weight.
-// xi_raw = (atan(t/U) - dLeft)/dCol + dShift, with U = R - x cos b - y sin b and
-// t = x sin b - y cos b (BP/transform.cu:82-85 with the negated sin/cos of :116).
-template <bool kWeighted>
-__device__ __forceinline__ float bp_sample(const float *P, int nCols, float R, float x, float y, float cb,
-                                           float sb, float xi_scale, float xi_offset)
+struct PixelGeo {
+    float x, y;     // FP32 pixel centre (mm), as the reference computes it
+    double xd, yd;  // FP64 pixel centre for edge decisions
+};
+
+__device__ __forceinline__ PixelGeo pixel_geo(const BpArgs &A, int i, int j)
 {
-    const float U = fmaf(-y, sb, fmaf(-x, cb, R));
-    const float t = fmaf(x, sb, -y * cb);
-    const float xi = fmaf(atanf(t * __builtin_amdgcn_rcpf(U)), xi_scale, xi_offset);
-    if (!(xi > 0.0f && xi < (float)nCols)) return 0.0f;
-    // texel-centre convention: interpolate at xi - 0.5; clamp to [0, nCols-1]
+    PixelGeo p;
+    p.x = fmaf(A.dx, (float)i + 0.5f, A.xmin);
+    p.y = fmaf(A.dy, (float)j + 0.5f, A.ymin);
+    p.xd = (double)A.xmin + (double)A.dx * ((double)i + 0.5);
+    p.yd = (double)A.ymin + (double)A.dy * ((double)j + 0.5);
+    return p;
+}
+
+// Column coordinate xi of the pixel in view v (xi_raw of BP/transform.cu:85), its
+// in-range flag and the FP32 U, t.
+__device__ __forceinline__ bool column_of(const BpArgs &A, const PixelGeo &p, int v, float &xi, float &U, float &t)
+{
+    const float2 cs = A.cs[v];
+    U = fmaf(-p.y, cs.y, fmaf(-p.x, cs.x, A.R));  // projection onto the iso-ray
+    t = fmaf(p.x, cs.y, -p.y * cs.x);             // distance to the centre ray
+    xi = fmaf(atanf(t * __builtin_amdgcn_rcpf(U)), A.xi_scale, A.xi_offset);
+    bool in = xi > 0.0f && xi < (float)A.nCols;
+    if (A.edge_exact && (fabsf(xi) < kEdgeBand || fabsf(xi - (float)A.nCols) < kEdgeBand)) {
+        const double2 c = A.csd[v];
+        const double Ud = (double)A.R - p.xd * c.x - p.yd * c.y;
+        const double td = p.xd * c.y - p.yd * c.x;
+        in = (td - Ud * A.tan_min > 0.0) && (td - Ud * A.tan_max < 0.0);
+    }
+    return in;
+}
+
+// Texel-centre linear interpolation position: interpolate at xi - 1/2, clamped to the
+// row; returns the left tap c0 in [0, nCols-2] and the weight of c0+1.
+__device__ __forceinline__ int interp_pos(float xi, int nCols, float &al)
+{
     const float ct = fminf(fmaxf(xi - 0.5f, 0.0f), (float)(nCols - 1));
     const float fc = fminf(floorf(ct), (float)max(nCols - 2, 0));
-    const int c0 = (int)fc;
-    const float al = ct - fc;
-    const float p0 = P[c0];
-    const float p1 = P[min(c0 + 1, nCols - 1)];
-    const float val = fmaf(al, p1 - p0, p0);
-    if (kWeighted) return val * __builtin_amdgcn_rcpf(fmaf(U, U, t * t));
-    return val;
+    al = ct - fc;
+    return (int)fc;
 }
 
 template <bool kWeighted>
@@ -59,15 +88,20 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A)
     const int j = blockIdx.y * kBpTy + threadIdx.x / kBpTx;
     const int sl = blockIdx.z;
     if (i >= A.nX || j >= A.nY) return;
-    const float x = fmaf(A.dx, (float)i + 0.5f, A.xmin);
-    const float y = fmaf(A.dy, (float)j + 0.5f, A.ymin);
+    const PixelGeo pg = pixel_geo(A, i, j);
     float acc = 0.0f;
-    if (x * x + y * y <= A.r2) {
+    if (pg.x * pg.x + pg.y * pg.y <= A.r2) {
         const float *sino = A.sino + (size_t)sl * A.sino_stride;
         for (int v = 0; v < A.nViews; ++v) {
-            const float2 cs = A.cs[v];
-            acc += bp_sample<kWeighted>(sino + (size_t)v * A.nCols, A.nCols, A.R, x, y, cs.x, cs.y, A.xi_scale,
-                                        A.xi_offset);
+            float xi, U, t;
+            if (!column_of(A, pg, v, xi, U, t)) continue;
+            float al;
+            const int c0 = interp_pos(xi, A.nCols, al);
+            const float *P = sino + (size_t)v * A.nCols;
+            const float p0 = P[c0];
+            const float p1 = P[min(c0 + 1, A.nCols - 1)];
+            const float val = fmaf(al, p1 - p0, p0);
+            acc += kWeighted ? val * __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : val;
         }
     }
     A.img[(size_t)sl * A.img_stride + (size_t)j * A.nX + i] = acc;
@@ -83,24 +117,18 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(BpArgs A)
     const int v = blockIdx.x, sl = blockIdx.z;
     for (int c = threadIdx.x; c < A.nCols; c += 256) row[c] = 0.0f;
     __syncthreads();
-    const float2 cs = A.cs[v];
     const float *img = A.img + (size_t)sl * A.img_stride;
     const int npix = A.nX * A.nY;
     for (int p = threadIdx.x; p < npix; p += 256) {
         const int i = p % A.nX, j = p / A.nX;
-        const float x = fmaf(A.dx, (float)i + 0.5f, A.xmin);
-        const float y = fmaf(A.dy, (float)j + 0.5f, A.ymin);
-        if (!(x * x + y * y <= A.r2)) continue;
+        const PixelGeo pg = pixel_geo(A, i, j);
+        if (!(pg.x * pg.x + pg.y * pg.y <= A.r2)) continue;
         const float val0 = img[p];
         if (val0 == 0.0f) continue;
-        const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
-        const float t = fmaf(x, cs.y, -y * cs.x);
-        const float xi = fmaf(atanf(t * __builtin_amdgcn_rcpf(U)), A.xi_scale, A.xi_offset);
-        if (!(xi > 0.0f && xi < (float)A.nCols)) continue;
-        const float ct = fminf(fmaxf(xi - 0.5f, 0.0f), (float)(A.nCols - 1));
-        const float fc = fminf(floorf(ct), (float)max(A.nCols - 2, 0));
-        const int c0 = (int)fc;
-        const float al = ct - fc;
+        float xi, U, t;
+        if (!column_of(A, pg, v, xi, U, t)) continue;
+        float al;
+        const int c0 = interp_pos(xi, A.nCols, al);
         const float val = kWeighted ? val0 * __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : val0;
         const float w1 = val * al;
         atomicAdd(&row[c0], val - w1);
@@ -117,6 +145,7 @@ BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, f
     A.sino = sino;
     A.img = img;
     A.cs = t.view_cs;
+    A.csd = t.view_csd;
     A.img_stride = (long long)p.g.nX * p.g.nY;
     A.sino_stride = (long long)p.g.nViews * p.g.nCols;
     A.nX = p.g.nX;
@@ -131,6 +160,9 @@ BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, f
     A.r2 = p.d.r2;
     A.xi_scale = p.d.bp_xi_scale;
     A.xi_offset = p.d.bp_xi_offset;
+    A.edge_exact = p.d.bp_edge_exact;
+    A.tan_min = p.d.bp_tan_min;
+    A.tan_max = p.d.bp_tan_max;
     return A;
 }
 

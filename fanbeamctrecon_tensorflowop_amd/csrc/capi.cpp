@@ -67,6 +67,25 @@ fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, in
     return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, false, static_cast<hipStream_t>(stream));
 }
 
+fbct_status fbct_fbp_backproject(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp_backproject");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_bp(*plan, *t, sino, image, n_slices, true, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
+                                         void *stream)
+{
+    fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fbp_backproject_adjoint");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, true, static_cast<hipStream_t>(stream));
+}
+
 fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows, void *stream)
 {
     if (!plan) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: null plan");

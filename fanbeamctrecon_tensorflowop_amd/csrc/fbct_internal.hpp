@@ -18,18 +18,16 @@ constexpr int kFpLanesPerRay = 16;
 constexpr int kFpRays = kFpThreads / kFpLanesPerRay;
 constexpr int kFpSlab = 32;
 
-// FP adjoint output tile (see fp_adjoint in fp.hip).
-constexpr int kFpaTile = 32;
-
 // Device-resident copy of a plan's tables on one HIP device.
 struct DeviceTables {
     int device = -1;
     int32_t *ray_n = nullptr;      // [nViews*nCols]
     float4 *ray_geo = nullptr;     // [nViews*nCols] (u0, w0, du, dw)
+    float2 *ray_inv = nullptr;     // [nViews*nCols] (1/du, 1/dw)
     uint8_t *fp_major = nullptr;   // [nViews*nFpBundles] 1 = x-major
     float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
+    double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
     double *ramp = nullptr;        // [2*nCols-1]
-    int32_t *fpa_cols = nullptr;   // [nTiles*nViews*2] candidate column range per tile/view
 };
 
 // Scalars the kernels need, derived once in FP64 on the host.
@@ -42,7 +40,15 @@ struct Derived {
     float bp_xi_scale;    // 1/dCol
     float bp_xi_offset;   // dShift - dLeft/dCol
     float r2;             // rFOV*rFOV (float, as the reference)
-    int fpa_tiles_x, fpa_tiles_y;
+    // In-range decision 0 < xi < nCols taken in FP64 near the two edges: with
+    // g = atan(t/U), the pixel is inside iff tan_min < t/U < tan_max.
+    int bp_edge_exact;    // 1 when the fan edges lie within (-pi/2, pi/2)
+    double bp_tan_min, bp_tan_max;
+    // FP adjoint (gather)
+    float fpa_r;          // rFOV/dx
+    float fpa_c_scale;    // 1/dCol
+    float fpa_c_off;      // -dLeft/dCol - 1/2
+    float fpa_far_x, fpa_far_y;  // 2r - n + 1/2: sample reach beyond the last pixel centre
 };
 
 }  // namespace fbct
@@ -54,10 +60,11 @@ struct fbct_plan {
     // host tables (FP64 computed, stored at the precision the kernels read)
     std::vector<int32_t> ray_n;
     std::vector<float> ray_geo;     // 4 per ray
+    std::vector<float> ray_inv;     // 2 per ray
     std::vector<uint8_t> fp_major;
     std::vector<float> view_cs;     // 2 per view
+    std::vector<double> view_csd;   // 2 per view
     std::vector<double> ramp;       // 2*nCols-1
-    std::vector<int32_t> fpa_cols;  // 2 per (tile, view)
     int64_t fp_samples = 0;
     fbct::Derived d{};
 

@@ -1,0 +1,172 @@
+// fp_adjoint.hip -- matched adjoint of the ray-driven FP, gathered per pixel (gfx950).
+//
+// The FP (fp.hip; reference FP/transform.cu:103-146) forms
+//     p[v][c] = 0.125 dx * sum_k  sum_taps  w_tap(u_k, w_k) * img[clamp(tap)]
+// with u_k = fma(k, du, u0), w_k = fma(k, dw, w0) in FP32.  Its transpose is
+//     img[j][i] = 0.125 dx * sum_v sum_c p[v][c] * W_c(i, j),
+//     W_c(i, j) = sum_k  T_i(u_k) * T_j(w_k),
+// where T_i(u) = max(0, 1 - |clamp(u, 0, nX-1) - i|) is the total bilinear weight the
+// clamped taps of a sample give pixel i (the clamp folds the taps -1 and nX, and any
+// sample beyond the image edge, into the border pixels).
+//
+// Each thread owns one output pixel and accumulates over all views, so every pixel is
+// written exactly once and no atomics are needed.  Per view, the candidate rays are
+// the columns whose lines pass within the support of the pixel (radius rho around its
+// centre, rho = sqrt(2) inside the image), found from the pixel's fan angle; per
+// candidate ray the sample range whose positions fall in the support is computed in
+// closed form and the samples are evaluated at exactly the FP32 positions the forward
+// kernel interpolates at.  A group of S slices that share the geometry is processed
+// together: the per-(pixel, view, ray) weight W_c is computed once and applied to S
+// sinograms.  Sums over views accumulate in FP64.
+
+#include <hip/hip_runtime.h>
+
+#include "fbct_internal.hpp"
+
+namespace fbct {
+namespace {
+
+constexpr int kAdjTx = 16, kAdjTy = 16;
+constexpr float kBig = 1e30f;
+
+struct FpaArgs {
+    const float *sino;
+    float *img;
+    const float4 *ray_geo;
+    const int32_t *ray_n;
+    const float2 *ray_inv;  // (1/du, 1/dw), 1e30 for a zero step
+    const float2 *cs;       // (cos beta, sin beta)
+    long long img_stride, sino_stride;
+    int nX, nY, nViews, nCols, nSlices;
+    float scale;            // 0.125 dx
+    float r;                // rFOV/dx: FP texel coordinate of the image centre is r - 1/2
+    float dx, R;
+    float c_scale, c_off;   // column coordinate c_f = atan(t/U) * c_scale + c_off (ray c at c_f = c)
+    float c_abs;            // |1/dCol|: radians -> columns
+    float far_x, far_y;     // how far samples reach beyond the last pixel centre (texels)
+};
+
+template <int S>
+__global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
+{
+    const int i = blockIdx.x * kAdjTx + threadIdx.x % kAdjTx;
+    const int j = blockIdx.y * kAdjTy + threadIdx.x / kAdjTx;
+    const int s0 = blockIdx.z * S;
+    if (i >= A.nX || j >= A.nY) return;
+
+    const float fi = (float)i, fj = (float)j;
+    // pixel centre in mm, in the FP's frame (dx on both axes, centre at texel r - 1/2)
+    const float x = (fi + 0.5f - A.r) * A.dx;
+    const float y = (fj + 0.5f - A.r) * A.dx;
+    // clamp window of the sample coordinate relative to this pixel (border plateaus)
+    const float xlo_c = (i == 0) ? 0.0f : -kBig, xhi_c = (i == A.nX - 1) ? 0.0f : kBig;
+    const float ylo_c = (j == 0) ? 0.0f : -kBig, yhi_c = (j == A.nY - 1) ? 0.0f : kBig;
+    // support of T_i in (u - i): (-1, 1), unbounded on the outer side of border pixels
+    const float xlo_b = (i == 0) ? -kBig : -1.0f, xhi_b = (i == A.nX - 1) ? kBig : 1.0f;
+    const float ylo_b = (j == 0) ? -kBig : -1.0f, yhi_b = (j == A.nY - 1) ? kBig : 1.0f;
+    const float ex = (i == A.nX - 1) ? fmaxf(1.0f, A.far_x) : 1.0f;
+    const float ey = (j == A.nY - 1) ? fmaxf(1.0f, A.far_y) : 1.0f;
+    const float rho = sqrtf(ex * ex + ey * ey);
+
+    const float *sino[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) sino[s] = A.sino + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+
+    double acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0;
+
+    for (int v = 0; v < A.nViews; ++v) {
+        const float2 cs = A.cs[v];
+        const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
+        const float t = fmaf(x, cs.y, -y * cs.x);
+        const float rU = __builtin_amdgcn_rcpf(U);
+        const float cf = fmaf(atanf(t * rU), A.c_scale, A.c_off);
+        // Rays within distance rho (texels) of the centre subtend at most asin(rho dx / L)
+        // from the pixel's fan angle; L >= U and asin(z) <= z (1 + z^2) for z <= 0.9.
+        const float z = rho * A.dx * rU;
+        const float hw = z < 0.9f ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs, 1e-3f) : 1e9f;
+        const int clo = (int)fmaxf(ceilf(cf - hw), 0.0f);
+        const int chi = (int)fminf(floorf(cf + hw), (float)(A.nCols - 1));
+        float vsum[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) vsum[s] = 0.0f;
+        const size_t rbase = (size_t)v * A.nCols;
+        for (int c = clo; c <= chi; ++c) {
+            const int N = A.ray_n[rbase + c];
+            if (N <= 0) continue;
+            const float4 g = A.ray_geo[rbase + c];
+            const float2 inv = A.ray_inv[rbase + c];
+            const float X0 = g.x - fi, Y0 = g.y - fj;
+            const float a1 = (xlo_b - X0) * inv.x, a2 = (xhi_b - X0) * inv.x;
+            const float b1 = (ylo_b - Y0) * inv.y, b2 = (yhi_b - Y0) * inv.y;
+            const float lo = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), 0.0f);
+            const float hi = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), (float)(N - 1));
+            if (!(lo <= hi)) continue;
+            const int k0 = (int)ceilf(lo), k1 = (int)floorf(hi);
+            float wsum = 0.0f;
+            for (int k = k0; k <= k1; ++k) {
+                const float kf = (float)k;
+                const float X = fmaf(kf, g.z, g.x) - fi;  // same FP32 position as the forward
+                const float Y = fmaf(kf, g.w, g.y) - fj;
+                const float wx = 1.0f - fabsf(fminf(fmaxf(X, xlo_c), xhi_c));
+                const float wy = 1.0f - fabsf(fminf(fmaxf(Y, ylo_c), yhi_c));
+                wsum = fmaf(fmaxf(wx, 0.0f), fmaxf(wy, 0.0f), wsum);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sino[s][rbase + c], vsum[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s];
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (s0 + s < A.nSlices)
+            A.img[(size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i] = (float)(acc[s] * (double)A.scale);
+}
+
+}  // namespace
+
+fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img, int n,
+                              hipStream_t stream)
+{
+    FpaArgs A;
+    A.sino = sino;
+    A.img = img;
+    A.ray_geo = t.ray_geo;
+    A.ray_n = t.ray_n;
+    A.ray_inv = t.ray_inv;
+    A.cs = t.view_cs;
+    A.img_stride = (long long)p.g.nX * p.g.nY;
+    A.sino_stride = (long long)p.g.nViews * p.g.nCols;
+    A.nX = p.g.nX;
+    A.nY = p.g.nY;
+    A.nViews = p.g.nViews;
+    A.nCols = p.g.nCols;
+    A.nSlices = n;
+    A.scale = p.d.fp_scale;
+    A.r = p.d.fpa_r;
+    A.dx = p.g.dx;
+    A.R = p.g.R;
+    A.c_scale = p.d.fpa_c_scale;
+    A.c_off = p.d.fpa_c_off;
+    A.c_abs = fabsf(p.d.fpa_c_scale);
+    A.far_x = p.d.fpa_far_x;
+    A.far_y = p.d.fpa_far_y;
+    const dim3 block(kAdjTx * kAdjTy);
+    const unsigned gx = (p.g.nX + kAdjTx - 1) / kAdjTx, gy = (p.g.nY + kAdjTy - 1) / kAdjTy;
+    // Slices that share the geometry are processed together when there are enough
+    // workgroups left to fill the device.
+    const long long tiles = (long long)gx * gy;
+    if (n >= 4 && tiles * ((n + 3) / 4) >= 1024)
+        hipLaunchKernelGGL(fp_adjoint_kernel<4>, dim3(gx, gy, (n + 3) / 4), block, 0, stream, A);
+    else if (n >= 2 && tiles * ((n + 1) / 2) >= 1024)
+        hipLaunchKernelGGL(fp_adjoint_kernel<2>, dim3(gx, gy, (n + 1) / 2), block, 0, stream, A);
+    else
+        hipLaunchKernelGGL(fp_adjoint_kernel<1>, dim3(gx, gy, n), block, 0, stream, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_adjoint_kernel launch: %s", hipGetErrorString(e));
+    return FBCT_OK;
+}
+
+}  // namespace fbct

@@ -139,15 +139,6 @@ static RayGeo ray_geometry(const fbct_fan_geom &g, double beta, int c)
     return out;
 }
 
-// Fan angle of a point (x, y) in mm as seen from the source of view (cb, sb):
-// BP/transform.cu:82-85 with the negated sin/cos passed at :116.
-static double fan_angle(const fbct_fan_geom &g, double cb, double sb, double x, double y)
-{
-    const double U = static_cast<double>(g.R) - x * cb - y * sb;
-    const double t = x * sb - y * cb;
-    return std::atan2(t, U);
-}
-
 static void build_tables(fbct_plan &p)
 {
     const fbct_fan_geom &g = p.g;
@@ -157,12 +148,16 @@ static void build_tables(fbct_plan &p)
 
     p.ray_n.assign(static_cast<size_t>(nV) * nC, 0);
     p.ray_geo.assign(static_cast<size_t>(nV) * nC * 4, 0.0f);
+    p.ray_inv.assign(static_cast<size_t>(nV) * nC * 2, 0.0f);
     p.view_cs.assign(static_cast<size_t>(nV) * 2, 0.0f);
+    p.view_csd.assign(static_cast<size_t>(nV) * 2, 0.0);
     int64_t samples = 0;
     for (int v = 0; v < nV; ++v) {
         const double beta = static_cast<double>(p.angles[v]);
-        p.view_cs[2 * v] = static_cast<float>(std::cos(beta));
-        p.view_cs[2 * v + 1] = static_cast<float>(std::sin(beta));
+        p.view_csd[2 * v] = std::cos(beta);
+        p.view_csd[2 * v + 1] = std::sin(beta);
+        p.view_cs[2 * v] = static_cast<float>(p.view_csd[2 * v]);
+        p.view_cs[2 * v + 1] = static_cast<float>(p.view_csd[2 * v + 1]);
         for (int c = 0; c < nC; ++c) {
             const RayGeo rg = ray_geometry(g, beta, c);
             const size_t k = static_cast<size_t>(v) * nC + c;
@@ -171,6 +166,10 @@ static void build_tables(fbct_plan &p)
             p.ray_geo[4 * k + 1] = static_cast<float>(rg.w0);
             p.ray_geo[4 * k + 2] = static_cast<float>(rg.du);
             p.ray_geo[4 * k + 3] = static_cast<float>(rg.dw);
+            // reciprocal of the FP32 step the kernels use; a zero step gets a large finite value
+            const float du = p.ray_geo[4 * k + 2], dw = p.ray_geo[4 * k + 3];
+            p.ray_inv[2 * k + 0] = du != 0.0f ? static_cast<float>(1.0 / static_cast<double>(du)) : 1e30f;
+            p.ray_inv[2 * k + 1] = dw != 0.0f ? static_cast<float>(1.0 / static_cast<double>(dw)) : 1e30f;
             samples += rg.n;
         }
     }
@@ -242,53 +241,25 @@ static void build_tables(fbct_plan &p)
         p.ramp[n + nC - 1] = h * kappa;
     }
 
-    // FP adjoint: for each output tile and view, the columns whose rays can touch the
-    // tile's taps (texel range [t0-1, t0+T] per axis), from the fan angles of the tile
-    // corners (convex region; the source lies outside it), with one column of margin.
-    d.fpa_tiles_x = (g.nX + kFpaTile - 1) / kFpaTile;
-    d.fpa_tiles_y = (g.nY + kFpaTile - 1) / kFpaTile;
-    const int ntiles = d.fpa_tiles_x * d.fpa_tiles_y;
-    p.fpa_cols.assign(static_cast<size_t>(ntiles) * nV * 2, 0);
-    const double dxd = static_cast<double>(g.dx);
-    for (int ty = 0; ty < d.fpa_tiles_y; ++ty)
-        for (int tx = 0; tx < d.fpa_tiles_x; ++tx) {
-            const double ua = tx * kFpaTile - 1.0, ub = std::min(g.nX, (tx + 1) * kFpaTile) + 0.0;
-            const double wa = ty * kFpaTile - 1.0, wb = std::min(g.nY, (ty + 1) * kFpaTile) + 0.0;
-            // texel-centre coordinate u maps to x = (u + 0.5 - r) dx (FP uses dx for both axes)
-            const double xs[2] = {(ua + 0.5 - r) * dxd, (ub + 0.5 - r) * dxd};
-            const double ys[2] = {(wa + 0.5 - r) * dxd, (wb + 0.5 - r) * dxd};
-            const double corner_r = std::max(std::hypot(xs[0], ys[0]), std::max(std::hypot(xs[1], ys[0]),
-                                    std::max(std::hypot(xs[0], ys[1]), std::hypot(xs[1], ys[1]))));
-            const bool convex_ok = corner_r < 0.999 * static_cast<double>(g.R);
-            const int tile = ty * d.fpa_tiles_x + tx;
-            for (int v = 0; v < nV; ++v) {
-                int32_t *cr = &p.fpa_cols[(static_cast<size_t>(tile) * nV + v) * 2];
-                if (!convex_ok) {
-                    cr[0] = 0;
-                    cr[1] = nC - 1;
-                    continue;
-                }
-                const double beta = static_cast<double>(p.angles[v]);
-                const double cb = std::cos(beta), sb = std::sin(beta);
-                double gmin = 1e300, gmax = -1e300;
-                for (double x : xs)
-                    for (double y : ys) {
-                        const double ga = fan_angle(g, cb, sb, x, y);
-                        gmin = std::min(gmin, ga);
-                        gmax = std::max(gmax, ga);
-                    }
-                double ca = (gmin - static_cast<double>(g.dLeft)) / dc - 0.5;
-                double cb2 = (gmax - static_cast<double>(g.dLeft)) / dc - 0.5;
-                if (ca > cb2) std::swap(ca, cb2);
-                const double lo = std::floor(ca) - 1.0, hi = std::ceil(cb2) + 1.0;
-                cr[0] = static_cast<int32_t>(std::max(0.0, std::min(lo, nC - 1.0)));
-                cr[1] = static_cast<int32_t>(std::max(-1.0, std::min(hi, nC - 1.0)));
-                if (hi < 0.0 || lo > nC - 1.0) {
-                    cr[0] = 0;
-                    cr[1] = -1;  // empty
-                }
-            }
-        }
+    // BP in-range test 0 < xi < nCols <=> g in (dLeft - dShift dCol, dLeft + (nCols - dShift) dCol)
+    // (either order when dCol < 0); near the edges it is decided on tangents in FP64.
+    {
+        const double g0 = static_cast<double>(g.dLeft) - static_cast<double>(g.dShift) * dc;
+        const double g1 = static_cast<double>(g.dLeft) + (nC - static_cast<double>(g.dShift)) * dc;
+        const double gmin = std::min(g0, g1), gmax = std::max(g0, g1);
+        const double lim = 1.5707963267948966 - 1e-6;
+        d.bp_edge_exact = (gmin > -lim && gmax < lim) ? 1 : 0;
+        d.bp_tan_min = d.bp_edge_exact ? std::tan(gmin) : 0.0;
+        d.bp_tan_max = d.bp_edge_exact ? std::tan(gmax) : 0.0;
+    }
+
+    // FP adjoint (gather): ray c passes through a point at column coordinate
+    // (g - dLeft)/dCol - 1/2 = c, the FP ignoring dShift.
+    d.fpa_r = static_cast<float>(r);
+    d.fpa_c_scale = static_cast<float>(1.0 / dc);
+    d.fpa_c_off = static_cast<float>(-static_cast<double>(g.dLeft) / dc - 0.5);
+    d.fpa_far_x = static_cast<float>(2.0 * r - g.nX + 0.5);
+    d.fpa_far_y = static_cast<float>(2.0 * r - g.nY + 0.5);
 }
 
 template <typename T>
@@ -312,9 +283,10 @@ static void free_tables(DeviceTables &t)
     (void)hipFree(t.ray_n);
     (void)hipFree(t.ray_geo);
     (void)hipFree(t.fp_major);
+    (void)hipFree(t.ray_inv);
     (void)hipFree(t.view_cs);
+    (void)hipFree(t.view_csd);
     (void)hipFree(t.ramp);
-    (void)hipFree(t.fpa_cols);
     if (cur >= 0) (void)hipSetDevice(cur);
     t = DeviceTables{};
 }
@@ -336,9 +308,10 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
     if ((s = upload(&t.ray_n, plan->ray_n.data(), plan->ray_n.size() * sizeof(int32_t))) != FBCT_OK ||
         (s = upload(&t.ray_geo, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.fp_major, plan->fp_major.data(), plan->fp_major.size())) != FBCT_OK ||
+        (s = upload(&t.ray_inv, plan->ray_inv.data(), plan->ray_inv.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
-        (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK ||
-        (s = upload(&t.fpa_cols, plan->fpa_cols.data(), plan->fpa_cols.size() * sizeof(int32_t))) != FBCT_OK) {
+        (s = upload(&t.view_csd, plan->view_csd.data(), plan->view_csd.size() * sizeof(double))) != FBCT_OK ||
+        (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK) {
         std::string msg = g_last_error;
         free_tables(t);
         g_last_error = msg;

@@ -145,6 +145,15 @@ class FanBeamPlan:
     def bp_adjoint(self, image: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_bp_adjoint", self._prep(image, self.image_shape, "bp_adjoint"), self.sino_shape)
 
+    def fbp_weighted_bp(self, filtered_sino: torch.Tensor) -> torch.Tensor:
+        """The 1/L^2-weighted backprojection step of FBP on an already filtered sinogram."""
+        return self._run("fbct_fbp_backproject", self._prep(filtered_sino, self.sino_shape, "fbp_weighted_bp"),
+                         self.image_shape)
+
+    def fbp_weighted_bp_adjoint(self, image: torch.Tensor) -> torch.Tensor:
+        return self._run("fbct_fbp_backproject_adjoint",
+                         self._prep(image, self.image_shape, "fbp_weighted_bp_adjoint"), self.sino_shape)
+
     def fbp(self, sino: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_fbp", self._prep(sino, self.sino_shape, "fbp"), self.image_shape, extra_ws=True)
 

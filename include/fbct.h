@@ -153,6 +153,15 @@ size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices);
 fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_slices,
                      void *workspace, size_t workspace_bytes, void *stream);
 
+/* The backprojection step of FBP alone: 1/L^2-weighted pixel-driven BP of an already
+ * filtered sinogram (FBP/transform.cu:61-89,182-188).  sino -> image. */
+fbct_status fbct_fbp_backproject(fbct_plan *plan, const float *filtered_sino, float *image, int n_slices,
+                                 void *stream);
+
+/* Transpose of fbct_fbp_backproject.  image -> sino.  No reference counterpart. */
+fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
+                                         void *stream);
+
 /* Exact adjoint of fbct_fbp.  image -> sino.  No reference counterpart. */
 fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
                              void *workspace, size_t workspace_bytes, void *stream);
