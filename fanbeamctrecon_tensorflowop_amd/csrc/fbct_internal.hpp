@@ -14,7 +14,7 @@ namespace fbct {
 
 // FP ray-march tiling (see fp.hip): rays per workgroup bundle and slab width in pixels.
 constexpr int kFpThreads = 256;
-constexpr int kFpLanesPerRay = 16;
+constexpr int kFpLanesPerRay = 4;
 constexpr int kFpRays = kFpThreads / kFpLanesPerRay;
 constexpr int kFpSlab = 32;
 

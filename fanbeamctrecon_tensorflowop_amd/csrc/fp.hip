@@ -35,10 +35,25 @@ struct FpArgs {
     const uint8_t *major;
     long long img_stride;
     long long sino_stride;
-    int nX, nY, nViews, nCols, nb;
+    int nX, nY, nViews, nCols, nb, nSlices;
     int nslab, cap;
     float scale;
 };
+
+// S interleaved slices of one tap: a native vector so one LDS read fetches all of them
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int S> struct Taps;
+template <> struct Taps<1> { using T = float; };
+template <> struct Taps<2> { using T = f32x2; };
+template <> struct Taps<4> { using T = f32x4; };
+
+template <int S>
+__device__ __forceinline__ float lane_of(const typename Taps<S>::T &t, int s)
+{
+    if constexpr (S == 1) return t;
+    else return t[s];
+}
 
 // Smallest k in [0, N] with P(k) true, where P(k) = (a0 + k da >= t) for da >= 0 and
 // (a0 + k da < t) for da < 0, evaluated with the same fmaf the samples use (monotone
@@ -77,14 +92,22 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
+// S slices that share the geometry are projected together: the LDS tile interleaves
+// them ([minor][major][slice]) so one sample position, its weights and its LDS
+// address serve S slices, with S-wide LDS reads.
+template <int S>
 __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 {
+    using TapT = typename Taps<S>::T;
+    // Dynamic LDS: the slab tile at address 0 (so tap offsets need no base add),
+    // followed by the per-wave min/max scratch red[2][2][waves].
     extern __shared__ float tile[];
-    __shared__ int red[2][2][kFpThreads / 64];
+    int(*red)[2][kFpThreads / 64] =
+        reinterpret_cast<int(*)[2][kFpThreads / 64]>(tile + (size_t)kPitch * A.cap * S);
 
     const int tid = threadIdx.x;
     const int lane = tid % kFpLanesPerRay;
-    const int b = blockIdx.x, v = blockIdx.y, sl = blockIdx.z;
+    const int b = blockIdx.x, v = blockIdx.y, s0 = blockIdx.z * S;
     const int c = b * kFpRays + tid / kFpLanesPerRay;
     const bool xmajor = A.major[(size_t)v * A.nb + b] != 0;
 
@@ -100,16 +123,27 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
         dm = xmajor ? g.w : g.z;
     }
     const bool asc = da >= 0.0f;
-    const float *img = A.img + (size_t)sl * A.img_stride;
+    const float *img[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) img[s] = A.img + (size_t)min(s0 + s, A.nSlices - 1) * A.img_stride;
     const int na = xmajor ? A.nX : A.nY;  // image extent along the major axis
     const int nm = xmajor ? A.nY : A.nX;
 
-    float acc = 0.0f;
+    // FP32 sums within a slab, FP64 across slabs
+    double accd[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) accd[s] = 0.0;
     int kprev = slab_boundary(a0, da, N, -1.0f);
     const int wave = tid / 64;
+    // byte strides of the interleaved tile
+    constexpr float kColB = 4.0f * S;
+    constexpr float kRowB = 4.0f * S * kPitch;
+    // LDS byte address of the tile (the dynamic-LDS base, folded into obase below)
+    const float tbase = (float)(unsigned)(size_t)((__attribute__((address_space(3))) float *)(tile));
+    using LdsTap = __attribute__((address_space(3))) const TapT;
 
-    for (int s = 0; s < A.nslab; ++s) {
-        const int alo = s * kFpSlab - 1;
+    for (int sl = 0; sl < A.nslab; ++sl) {
+        const int alo = sl * kFpSlab - 1;
         const int knext = slab_boundary(a0, da, N, (float)(alo + kFpSlab));
         const int ka = asc ? kprev : knext;
         const int kb = asc ? knext : kprev;
@@ -125,15 +159,15 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
         mn = wave_min(mn);
         mx = wave_max(mx);
         if ((tid & 63) == 0) {
-            red[s & 1][0][wave] = mn;
-            red[s & 1][1][wave] = mx;
+            red[sl & 1][0][wave] = mn;
+            red[sl & 1][1][wave] = mx;
         }
         __syncthreads();  // reductions visible; previous slab's reads of `tile` done
-        int mlo = red[s & 1][0][0], mhi = red[s & 1][1][0];
+        int mlo = red[sl & 1][0][0], mhi = red[sl & 1][1][0];
 #pragma unroll
         for (int w = 1; w < kFpThreads / 64; ++w) {
-            mlo = min(mlo, red[s & 1][0][w]);
-            mhi = max(mhi, red[s & 1][1][w]);
+            mlo = min(mlo, red[sl & 1][0][w]);
+            mhi = max(mhi, red[sl & 1][1][w]);
         }
         if (mlo > mhi) continue;  // no samples of this bundle in the slab (uniform)
         const int M = mhi - mlo + 1;
@@ -142,48 +176,70 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
             continue;
         }
 
-        // Stage taps [alo, alo+kFpSlab] x [mlo, mhi] with clamp-to-edge addressing.
-        // Global loads run along the image row (i) in both orientations.
+        // Stage taps [alo, alo+kFpSlab] x [mlo, mhi] of the S slices with clamp-to-edge
+        // addressing.  Global loads run along the image row (i) in both orientations.
         const int F = xmajor ? kPitch : M;  // extent of the fast (row-contiguous) index
         const int total = F * (xmajor ? M : kPitch);
         const float invF = 1.0f / (float)F;
         for (int idx = tid; idx < total; idx += kFpThreads) {
             const int sidx = (int)(((float)idx + 0.5f) * invF);
             const int fidx = idx - sidx * F;
-            int aa, mm;
-            if (xmajor) {
-                aa = fidx;
-                mm = sidx;
-            } else {
-                mm = fidx;
-                aa = sidx;
-            }
+            const int aa = xmajor ? fidx : sidx;
+            const int mm = xmajor ? sidx : fidx;
             const int ga = min(max(alo + aa, 0), na - 1);
             const int gm = min(max(mlo + mm, 0), nm - 1);
-            const int gi = xmajor ? ga : gm;
-            const int gj = xmajor ? gm : ga;
-            tile[mm * kPitch + aa] = img[(size_t)gj * A.nX + gi];
+            const size_t go = xmajor ? (size_t)gm * A.nX + ga : (size_t)ga * A.nX + gm;
+            TapT val;
+            float *vp = reinterpret_cast<float *>(&val);
+#pragma unroll
+            for (int s = 0; s < S; ++s) vp[s] = img[s][go];
+            reinterpret_cast<TapT *>(tile)[mm * kPitch + aa] = val;
         }
         __syncthreads();  // tile ready
 
-        for (int k = ka + lane; k < kb; k += kFpLanesPerRay) {
-            const float kf = (float)k;
+        // byte offset of tap (fa, fm) = (fm - mlo) * kRowB + (fa - alo) * kColB, exact in FP32
+        const float obase = tbase - ((float)mlo * kRowB + (float)alo * kColB);
+        const float kend = (float)kb;
+        float acc[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+        for (float kf = (float)(ka + lane); kf < kend; kf += (float)kFpLanesPerRay) {
             const float a = fmaf(kf, da, a0);
             const float m = fmaf(kf, dm, m0);
             const float fa = floorf(a), fm = floorf(m);
             const float ta = a - fa, tm = m - fm;
-            const int o = ((int)fm - mlo) * kPitch + ((int)fa - alo);
-            const float v00 = tile[o], v01 = tile[o + 1];
-            const float v10 = tile[o + kPitch], v11 = tile[o + kPitch + 1];
-            const float top = fmaf(ta, v01 - v00, v00);
-            const float bot = fmaf(ta, v11 - v10, v10);
-            acc += fmaf(tm, bot - top, top);
+            const unsigned o = (unsigned)fmaf(fm, kRowB, fmaf(fa, kColB, obase));
+            const TapT t00 = *reinterpret_cast<LdsTap *>((size_t)o);
+            const TapT t01 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)kColB));
+            const TapT t10 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)kRowB));
+            const TapT t11 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)(kRowB + kColB)));
+            if (S == 1) {
+                const float v00 = lane_of<S>(t00, 0), v01 = lane_of<S>(t01, 0);
+                const float v10 = lane_of<S>(t10, 0), v11 = lane_of<S>(t11, 0);
+                const float top = fmaf(ta, v01 - v00, v00);
+                const float bot = fmaf(ta, v11 - v10, v10);
+                acc[0] += fmaf(tm, bot - top, top);
+            } else {
+                const float ua = 1.0f - ta, um = 1.0f - tm;
+                const float w00 = ua * um, w01 = ta * um, w10 = ua * tm, w11 = ta * tm;
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    acc[s] = fmaf(w00, lane_of<S>(t00, s),
+                                  fmaf(w01, lane_of<S>(t01, s),
+                                       fmaf(w10, lane_of<S>(t10, s), fmaf(w11, lane_of<S>(t11, s), acc[s]))));
+            }
         }
+#pragma unroll
+        for (int s = 0; s < S; ++s) accd[s] += (double)acc[s];
     }
 
 #pragma unroll
-    for (int off = kFpLanesPerRay / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0 && c < A.nCols) A.sino[(size_t)sl * A.sino_stride + (size_t)v * A.nCols + c] = acc * A.scale;
+    for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int off = kFpLanesPerRay / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
+        if (lane == 0 && c < A.nCols && s0 + s < A.nSlices)
+            A.sino[(size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c] = (float)(accd[s] * (double)A.scale);
+    }
 }
 
 __device__ int g_fp_err = 0;
@@ -209,14 +265,24 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const float *im
     A.nslab = p.d.fp_nslab;
     A.cap = p.d.fp_cap;
     A.scale = p.d.fp_scale;
-    const size_t lds = (size_t)kPitch * p.d.fp_cap * sizeof(float);
-    if (lds > 160 * 1024 - 1024)
-        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds);
+    A.nSlices = n;
     int *err = nullptr;
     hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&err), HIP_SYMBOL(g_fp_err));
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetSymbolAddress: %s", hipGetErrorString(e));
-    dim3 grid(p.d.fp_nbundles, p.g.nViews, n);
-    hipLaunchKernelGGL(fp_kernel, grid, dim3(kFpThreads), lds, stream, A, err);
+    // Slices per workgroup: share each sample's position and weights across up to 4
+    // slices when the grid still holds enough workgroups and the tile fits in LDS.
+    const size_t lds1 = (size_t)kPitch * p.d.fp_cap * sizeof(float);
+    const size_t red_bytes = 2 * 2 * (kFpThreads / 64) * sizeof(int);
+    const long long wg1 = (long long)p.d.fp_nbundles * p.g.nViews;
+    int S = 1;
+    if (n >= 4 && lds1 * 4 <= 64 * 1024 && wg1 * ((n + 3) / 4) >= 2048) S = 4;
+    else if (n >= 2 && lds1 * 2 <= 64 * 1024 && wg1 * ((n + 1) / 2) >= 2048) S = 2;
+    if (lds1 * S + red_bytes > 160 * 1024)
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds1 * S);
+    dim3 grid(p.d.fp_nbundles, p.g.nViews, (n + S - 1) / S);
+    if (S == 4) hipLaunchKernelGGL(fp_kernel<4>, grid, dim3(kFpThreads), lds1 * 4 + red_bytes, stream, A, err);
+    else if (S == 2) hipLaunchKernelGGL(fp_kernel<2>, grid, dim3(kFpThreads), lds1 * 2 + red_bytes, stream, A, err);
+    else hipLaunchKernelGGL(fp_kernel<1>, grid, dim3(kFpThreads), lds1 + red_bytes, stream, A, err);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

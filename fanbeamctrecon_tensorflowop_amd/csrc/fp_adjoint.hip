@@ -46,27 +46,30 @@ struct FpaArgs {
     float far_x, far_y;     // how far samples reach beyond the last pixel centre (texels)
 };
 
-template <int S>
-__global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
+// Per-pixel accumulation over all views.  kBorder: the workgroup's tile touches the
+// image edge, so the clamp window and the unbounded outer support of border pixels
+// apply; interior tiles skip them.
+template <int S, bool kBorder>
+__device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, int s0)
 {
-    const int i = blockIdx.x * kAdjTx + threadIdx.x % kAdjTx;
-    const int j = blockIdx.y * kAdjTy + threadIdx.x / kAdjTx;
-    const int s0 = blockIdx.z * S;
-    if (i >= A.nX || j >= A.nY) return;
-
     const float fi = (float)i, fj = (float)j;
     // pixel centre in mm, in the FP's frame (dx on both axes, centre at texel r - 1/2)
     const float x = (fi + 0.5f - A.r) * A.dx;
     const float y = (fj + 0.5f - A.r) * A.dx;
-    // clamp window of the sample coordinate relative to this pixel (border plateaus)
-    const float xlo_c = (i == 0) ? 0.0f : -kBig, xhi_c = (i == A.nX - 1) ? 0.0f : kBig;
-    const float ylo_c = (j == 0) ? 0.0f : -kBig, yhi_c = (j == A.nY - 1) ? 0.0f : kBig;
-    // support of T_i in (u - i): (-1, 1), unbounded on the outer side of border pixels
-    const float xlo_b = (i == 0) ? -kBig : -1.0f, xhi_b = (i == A.nX - 1) ? kBig : 1.0f;
-    const float ylo_b = (j == 0) ? -kBig : -1.0f, yhi_b = (j == A.nY - 1) ? kBig : 1.0f;
-    const float ex = (i == A.nX - 1) ? fmaxf(1.0f, A.far_x) : 1.0f;
-    const float ey = (j == A.nY - 1) ? fmaxf(1.0f, A.far_y) : 1.0f;
-    const float rho = sqrtf(ex * ex + ey * ey);
+    float xlo_c = -kBig, xhi_c = kBig, ylo_c = -kBig, yhi_c = kBig;
+    float xlo_b = -1.0f, xhi_b = 1.0f, ylo_b = -1.0f, yhi_b = 1.0f;
+    float rho = 1.41421356f;
+    if (kBorder) {
+        // clamp window of the sample coordinate relative to this pixel (border plateaus)
+        // and support of T_i in (u - i), unbounded on the outer side of border pixels
+        if (i == 0) xlo_c = 0.0f, xlo_b = -kBig;
+        if (i == A.nX - 1) xhi_c = 0.0f, xhi_b = kBig;
+        if (j == 0) ylo_c = 0.0f, ylo_b = -kBig;
+        if (j == A.nY - 1) yhi_c = 0.0f, yhi_b = kBig;
+        const float ex = (i == A.nX - 1) ? fmaxf(1.0f, A.far_x) : 1.0f;
+        const float ey = (j == A.nY - 1) ? fmaxf(1.0f, A.far_y) : 1.0f;
+        rho = sqrtf(ex * ex + ey * ey);
+    }
 
     const float *sino[S];
 #pragma unroll
@@ -103,15 +106,19 @@ __global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
             const float lo = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), 0.0f);
             const float hi = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), (float)(N - 1));
             if (!(lo <= hi)) continue;
-            const int k0 = (int)ceilf(lo), k1 = (int)floorf(hi);
             float wsum = 0.0f;
-            for (int k = k0; k <= k1; ++k) {
-                const float kf = (float)k;
-                const float X = fmaf(kf, g.z, g.x) - fi;  // same FP32 position as the forward
-                const float Y = fmaf(kf, g.w, g.y) - fj;
-                const float wx = 1.0f - fabsf(fminf(fmaxf(X, xlo_c), xhi_c));
-                const float wy = 1.0f - fabsf(fminf(fmaxf(Y, ylo_c), yhi_c));
-                wsum = fmaf(fmaxf(wx, 0.0f), fmaxf(wy, 0.0f), wsum);
+            const float kend = floorf(hi);
+            for (float kf = ceilf(lo); kf <= kend; kf += 1.0f) {
+                const float X = fmaf(kf, g.z, X0);
+                const float Y = fmaf(kf, g.w, Y0);
+                if (kBorder) {
+                    const float wx = 1.0f - fabsf(fminf(fmaxf(X, xlo_c), xhi_c));
+                    const float wy = 1.0f - fabsf(fminf(fmaxf(Y, ylo_c), yhi_c));
+                    wsum = fmaf(fmaxf(wx, 0.0f), fmaxf(wy, 0.0f), wsum);
+                } else {
+                    // inside the support |X|, |Y| < 1 up to rounding at its two ends
+                    wsum = fmaf(1.0f - fabsf(X), 1.0f - fabsf(Y), wsum);
+                }
             }
 #pragma unroll
             for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sino[s][rbase + c], vsum[s]);
@@ -123,6 +130,21 @@ __global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
     for (int s = 0; s < S; ++s)
         if (s0 + s < A.nSlices)
             A.img[(size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i] = (float)(acc[s] * (double)A.scale);
+}
+
+template <int S>
+__global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
+{
+    const int i = blockIdx.x * kAdjTx + threadIdx.x % kAdjTx;
+    const int j = blockIdx.y * kAdjTy + threadIdx.x / kAdjTx;
+    const int s0 = blockIdx.z * S;
+    if (i >= A.nX || j >= A.nY) return;
+    const bool border_tile = blockIdx.x == 0 || blockIdx.y == 0 || (int)(blockIdx.x + 1) * kAdjTx >= A.nX ||
+                             (int)(blockIdx.y + 1) * kAdjTy >= A.nY;
+    if (border_tile)
+        adjoint_pixel<S, true>(A, i, j, s0);
+    else
+        adjoint_pixel<S, false>(A, i, j, s0);
 }
 
 }  // namespace
@@ -158,7 +180,9 @@ fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     // Slices that share the geometry are processed together when there are enough
     // workgroups left to fill the device.
     const long long tiles = (long long)gx * gy;
-    if (n >= 4 && tiles * ((n + 3) / 4) >= 1024)
+    if (n >= 8 && tiles * ((n + 7) / 8) >= 1024)
+        hipLaunchKernelGGL(fp_adjoint_kernel<8>, dim3(gx, gy, (n + 7) / 8), block, 0, stream, A);
+    else if (n >= 4 && tiles * ((n + 3) / 4) >= 1024)
         hipLaunchKernelGGL(fp_adjoint_kernel<4>, dim3(gx, gy, (n + 3) / 4), block, 0, stream, A);
     else if (n >= 2 && tiles * ((n + 1) / 2) >= 1024)
         hipLaunchKernelGGL(fp_adjoint_kernel<2>, dim3(gx, gy, (n + 1) / 2), block, 0, stream, A);

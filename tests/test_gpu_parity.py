@@ -124,10 +124,12 @@ def test_fp_adjoint_matches_oracle(small):
 def test_bp_adjoint_matches_oracle(small, weighted, image):
     # On a white-noise image every interpolation weight matters independently, so the
     # FP32 rounding of the column coordinate (~3e-5 column at 512 columns) shows up at
-    # the 1e-5 level; the tolerance for that stress input is 1e-4.
+    # the 1e-5 level; the tolerance for that stress input is 1e-4.  FBP^T applies the
+    # ramp filter after the FP32 scatter, which amplifies its rounding on smooth inputs
+    # as well (tolerance 1e-4).
     plan, o, img = small
     if image == "phantom":
-        x, tol = img.astype(np.float32), TOL
+        x, tol = img.astype(np.float32), (1e-4 if weighted else TOL)
     else:
         x, tol = np.random.default_rng(13).normal(size=(1, 256, 256)).astype(np.float32), 1e-4
     if weighted:
@@ -168,12 +170,17 @@ def test_adjoint_dot_product_clinical(clinical, op):
 
 # ---------------------------------------------------------------- edge cases
 
-def test_batch_equals_per_slice(small):
+@pytest.mark.parametrize("n", [2, 3, 5, 8])
+def test_batch_equals_per_slice(small, n):
+    # batches share each sample's position across slices (a different but equivalent
+    # bilinear arithmetic), so slices agree with single-slice calls to FP32 rounding
     plan, o, img = small
-    x = np.stack([img[0], np.roll(img[0], 17, axis=1), img[0][::-1].copy()])
-    b = host(plan.fp(dev(x)))
-    for s in range(3):
-        np.testing.assert_array_equal(b[s], host(plan.fp(dev(x[s:s + 1])))[0])
+    x = np.stack([np.roll(img[0], 17 * s, axis=1) for s in range(n)])
+    y = np.random.default_rng(n).normal(size=(n, 360, 512)).astype(np.float32)
+    bf, ba = host(plan.fp(dev(x))), host(plan.fp_adjoint(dev(y)))
+    for s in range(n):
+        assert rel_l2(bf[s], host(plan.fp(dev(x[s:s + 1])))[0]) <= 1e-6
+        assert rel_l2(ba[s], host(plan.fp_adjoint(dev(y[s:s + 1])))[0]) <= 1e-6
 
 
 def test_zero_slices():
