@@ -66,28 +66,42 @@ def main(argv=None):
     base = workloads.clinical_phantom(min(S, 8))
     img = np.concatenate([np.roll(base, 7 * k, axis=2) for k in range((S + 7) // 8)])[:S]
     x = torch.from_numpy(np.ascontiguousarray(img)).to(dev)
-    target = plan.fp(x) * 1.01
+    # target sinograms: projections of the phantoms shifted by 9 pixels (a residual of
+    # the size of the signal, as between a prediction and a measurement)
+    target = plan.fp(torch.roll(x, shifts=9, dims=2))
     torch.cuda.synchronize()
 
-    def step():
+    # The step: FP of the batch, residual against a target, matched adjoint of the
+    # residual.  BF16 storage: images and the residual sinogram are held in BF16 and
+    # widened on load (FP32 arithmetic, FP32 sinogram and gradient outputs).
+    xb = x.to(torch.bfloat16)
+    targetb = target
+
+    def step_f32():
         y = plan.fp(x)
         return plan.fp_adjoint(y - target)
 
-    log("bench: warmup %d" % args.warmup)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    def step_bf16():
+        y = plan.fp(xb)
+        return plan.fp_adjoint((y - targetb).to(torch.bfloat16))
 
-    log("bench: timing %d steps" % args.steps)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        step()
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / args.steps
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record()
+        for _ in range(args.steps):
+            out_ = step()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.steps, time.perf_counter() - t0, out_
+
+    log("bench: warmup %d, timing %d steps (BF16 storage)" % (args.warmup, args.steps))
+    ms_step, wall, g_bf16 = timed(step_bf16)
+    log("bench: FP32 storage step")
+    ms_step_f32, _, g_f32 = timed(step_f32)
     gups = 2.0 * S * U / (ms_step * 1e-3) / 1e9
 
     def time_op(fn, reps):
@@ -139,8 +153,11 @@ def main(argv=None):
         "data": "synthetic",
         "config": {
             "workload": "clinical fan-beam FP + matched adjoint, %d slices of 512x512, 984 views x 888 bins" % S,
-            "slices": S, "image": [nY, nX], "views": nV, "bins": nC, "storage": "f32",
+            "slices": S, "image": [nY, nX], "views": nV, "bins": nC,
+            "storage": "bf16 image and residual, f32 sinogram and gradient; f32 arithmetic",
         },
+        "ms_per_slice_batched_f32_storage": round(ms_step_f32 / S, 4),
+        "gups_batched_f32_storage": round(2.0 * S * U / (ms_step_f32 * 1e-3) / 1e9, 3),
         "fp_gups": round(S * U / (ms_fp * 1e-3) / 1e9, 3),
         "fp_adjoint_gups": round(S * U / (ms_fpa * 1e-3) / 1e9, 3),
         "bp_gups": round(S * U / (ms_bp * 1e-3) / 1e9, 3),
@@ -148,6 +165,7 @@ def main(argv=None):
         "ms_per_slice_batched": round(ms_step / S, 4),
         "ms_per_slice_plain": round(ms_plain / S, 4),
         "batched_vs_plain": round(ms_plain / ms_step, 3),
+        "batched_f32_storage_vs_plain": round(ms_plain / ms_step_f32, 3),
         "wall_s": round(wall, 3),
         "roofline": {
             "kernel": "fp_kernel",
@@ -173,8 +191,23 @@ def main(argv=None):
         one = img[:1]
         t0 = time.perf_counter()
         yo = o.fp(one, nthreads=nthr)
-        o.fp_adjoint(yo, nthreads=nthr)
+        go = o.fp_adjoint(yo - target[:1].cpu().numpy(), nthreads=nthr)
         tc = time.perf_counter() - t0
+        one_b = xb[:1].float().cpu().numpy()  # the BF16-stored image, widened
+        yb = o.fp(one_b, nthreads=nthr)
+        gb = o.fp_adjoint((yb - target[:1].cpu().numpy()).astype(np.float32).astype(np.float64), nthreads=nthr)
+        import numpy as _np
+
+        def _rel(a_, b_):
+            return float(_np.linalg.norm(a_ - b_) / _np.linalg.norm(b_))
+
+        out["error_vs_fp64_oracle"] = {
+            "note": "relative L2 of slice 0 against the FP64 oracle",
+            "step_f32_storage": _rel(g_f32[:1].double().cpu().numpy(), go),
+            "step_bf16_storage_vs_fp32_inputs": _rel(g_bf16[:1].double().cpu().numpy(), go),
+            "step_bf16_storage_vs_bf16_inputs": _rel(g_bf16[:1].double().cpu().numpy(), gb),
+            "fp_f32": _rel(plan.fp(x[:1]).double().cpu().numpy(), yo),
+        }
         out["cpu_baseline"] = {
             "value": round(2.0 * U / tc / 1e9, 4), "unit": "GUPS", "cores": nthr, "kind": "port",
             "sample": "FP64 oracle FP + matched adjoint of 1 of the %d slices (%.1f s)" % (S, tc),

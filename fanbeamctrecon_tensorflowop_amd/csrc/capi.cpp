@@ -22,6 +22,12 @@ fbct_status check_common(fbct_plan *plan, const void *in, const void *out, int64
     return FBCT_OK;
 }
 
+fbct_status check_dtype(int d, const char *what)
+{
+    if (d != FBCT_F32 && d != FBCT_BF16) return fail(FBCT_ERR_INVALID_ARGUMENT, "%s: unknown storage type %d", what, d);
+    return FBCT_OK;
+}
+
 size_t sino_bytes(const fbct_plan *p, int64_t n)
 {
     return static_cast<size_t>(n) * p->g.nViews * p->g.nCols * sizeof(float);
@@ -37,7 +43,20 @@ fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slic
     if (s != FBCT_OK || n_slices == 0) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::launch_fp(*plan, *t, image, sino, n_slices, static_cast<hipStream_t>(stream));
+    return fbct::launch_fp(*plan, *t, image, FBCT_F32, sino, FBCT_F32, n_slices, static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void *sino, int sino_dtype,
+                       int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fp_ex");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_dtype(image_dtype, "fbct_fp_ex")) != FBCT_OK || (s = check_dtype(sino_dtype, "fbct_fp_ex")) != FBCT_OK)
+        return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_fp(*plan, *t, image, static_cast<fbct_dtype>(image_dtype), sino,
+                           static_cast<fbct_dtype>(sino_dtype), n_slices, static_cast<hipStream_t>(stream));
 }
 
 fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream)
@@ -46,7 +65,22 @@ fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, in
     if (s != FBCT_OK || n_slices == 0) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::launch_fp_adjoint(*plan, *t, sino, image, n_slices, static_cast<hipStream_t>(stream));
+    return fbct::launch_fp_adjoint(*plan, *t, sino, FBCT_F32, image, FBCT_F32, n_slices,
+                                   static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_fp_adjoint_ex(fbct_plan *plan, const void *sino, int sino_dtype, void *image, int image_dtype,
+                               int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fp_adjoint_ex");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_dtype(sino_dtype, "fbct_fp_adjoint_ex")) != FBCT_OK ||
+        (s = check_dtype(image_dtype, "fbct_fp_adjoint_ex")) != FBCT_OK)
+        return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_fp_adjoint(*plan, *t, sino, static_cast<fbct_dtype>(sino_dtype), image,
+                                   static_cast<fbct_dtype>(image_dtype), n_slices, static_cast<hipStream_t>(stream));
 }
 
 fbct_status fbct_bp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream)

@@ -1,0 +1,43 @@
+// dtype.hpp -- storage types of images and sinograms in device memory.
+//
+// Arithmetic is always FP32 (sums in FP32/FP64); BF16 is a storage format only:
+// widened exactly on load, rounded to nearest even on store.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fbct {
+
+using bf16_t = uint16_t;  // raw bfloat16 bits
+
+__device__ __forceinline__ float load_f(const float *p, size_t i)
+{
+    return p[i];
+}
+
+__device__ __forceinline__ float load_f(const bf16_t *p, size_t i)
+{
+    return __uint_as_float(static_cast<unsigned>(p[i]) << 16);
+}
+
+__device__ __forceinline__ bf16_t f32_to_bf16(float v)
+{
+    const unsigned u = __float_as_uint(v);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))  // NaN stays a quiet NaN
+        return static_cast<bf16_t>((u >> 16) | 0x0040u);
+    return static_cast<bf16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ void store_f(float *p, size_t i, float v)
+{
+    p[i] = v;
+}
+
+__device__ __forceinline__ void store_f(bf16_t *p, size_t i, float v)
+{
+    p[i] = f32_to_bf16(v);
+}
+
+}  // namespace fbct

@@ -82,10 +82,10 @@ fbct_status ok();
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
 
 // Kernel launchers (fp.hip, bp.hip, filter.hip).  All asynchronous on `stream`.
-fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const float *img, float *sino,
-                      int n, hipStream_t stream);
-fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *sino,
-                              float *img, int n, hipStream_t stream);
+fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti,
+                      void *sino, fbct_dtype to, int n, hipStream_t stream);
+fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const void *sino,
+                              fbct_dtype ti, void *img, fbct_dtype to, int n, hipStream_t stream);
 fbct_status launch_bp(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img,
                       int n, bool weighted, hipStream_t stream);
 fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *img,

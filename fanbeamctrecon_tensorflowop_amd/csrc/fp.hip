@@ -20,6 +20,7 @@
 
 #include <climits>
 
+#include "dtype.hpp"
 #include "fbct_internal.hpp"
 
 namespace fbct {
@@ -28,8 +29,8 @@ namespace {
 constexpr int kPitch = kFpSlab + 1;  // LDS row: taps [alo, alo + kFpSlab]
 
 struct FpArgs {
-    const float *img;
-    float *sino;
+    const void *img;   // TI[n][nY][nX]
+    void *sino;        // TO[n][nViews][nCols]
     const float4 *ray_geo;
     const int32_t *ray_n;
     const uint8_t *major;
@@ -95,7 +96,7 @@ __device__ __forceinline__ int wave_max(int v)
 // S slices that share the geometry are projected together: the LDS tile interleaves
 // them ([minor][major][slice]) so one sample position, its weights and its LDS
 // address serve S slices, with S-wide LDS reads.
-template <int S>
+template <int S, typename TI, typename TO>
 __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 {
     using TapT = typename Taps<S>::T;
@@ -123,9 +124,10 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
         dm = xmajor ? g.w : g.z;
     }
     const bool asc = da >= 0.0f;
-    const float *img[S];
+    const TI *img[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) img[s] = A.img + (size_t)min(s0 + s, A.nSlices - 1) * A.img_stride;
+    for (int s = 0; s < S; ++s)
+        img[s] = static_cast<const TI *>(A.img) + (size_t)min(s0 + s, A.nSlices - 1) * A.img_stride;
     const int na = xmajor ? A.nX : A.nY;  // image extent along the major axis
     const int nm = xmajor ? A.nY : A.nX;
 
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
             TapT val;
             float *vp = reinterpret_cast<float *>(&val);
 #pragma unroll
-            for (int s = 0; s < S; ++s) vp[s] = img[s][go];
+            for (int s = 0; s < S; ++s) vp[s] = load_f(img[s], go);
             reinterpret_cast<TapT *>(tile)[mm * kPitch + aa] = val;
         }
         __syncthreads();  // tile ready
@@ -238,7 +240,8 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 #pragma unroll
         for (int off = kFpLanesPerRay / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
         if (lane == 0 && c < A.nCols && s0 + s < A.nSlices)
-            A.sino[(size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c] = (float)(accd[s] * (double)A.scale);
+            store_f(static_cast<TO *>(A.sino), (size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c,
+                    (float)(accd[s] * (double)A.scale));
     }
 }
 
@@ -246,8 +249,26 @@ __device__ int g_fp_err = 0;
 
 }  // namespace
 
-fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const float *img, float *sino, int n,
-                      hipStream_t stream)
+template <int S, typename TI, typename TO>
+static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, int *err)
+{
+    hipLaunchKernelGGL((fp_kernel<S, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, err);
+}
+
+template <int S>
+static fbct_status launch_fp_t(fbct_dtype ti, fbct_dtype to, dim3 grid, size_t lds, hipStream_t stream,
+                               const FpArgs &A, int *err)
+{
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, float, float>(grid, lds, stream, A, err);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, bf16_t, float>(grid, lds, stream, A, err);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, float, bf16_t>(grid, lds, stream, A, err);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, bf16_t, bf16_t>(grid, lds, stream, A, err);
+    else return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp: unknown storage type");
+    return FBCT_OK;
+}
+
+fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti, void *sino,
+                      fbct_dtype to, int n, hipStream_t stream)
 {
     FpArgs A;
     A.img = img;
@@ -280,9 +301,11 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const float *im
     if (lds1 * S + red_bytes > 160 * 1024)
         return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds1 * S);
     dim3 grid(p.d.fp_nbundles, p.g.nViews, (n + S - 1) / S);
-    if (S == 4) hipLaunchKernelGGL(fp_kernel<4>, grid, dim3(kFpThreads), lds1 * 4 + red_bytes, stream, A, err);
-    else if (S == 2) hipLaunchKernelGGL(fp_kernel<2>, grid, dim3(kFpThreads), lds1 * 2 + red_bytes, stream, A, err);
-    else hipLaunchKernelGGL(fp_kernel<1>, grid, dim3(kFpThreads), lds1 + red_bytes, stream, A, err);
+    fbct_status st;
+    if (S == 4) st = launch_fp_t<4>(ti, to, grid, lds1 * 4 + red_bytes, stream, A, err);
+    else if (S == 2) st = launch_fp_t<2>(ti, to, grid, lds1 * 2 + red_bytes, stream, A, err);
+    else st = launch_fp_t<1>(ti, to, grid, lds1 + red_bytes, stream, A, err);
+    if (st != FBCT_OK) return st;
     e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dtype.hpp"
 #include "fbct_internal.hpp"
 
 namespace fbct {
@@ -30,8 +31,8 @@ constexpr int kAdjTx = 16, kAdjTy = 16;
 constexpr float kBig = 1e30f;
 
 struct FpaArgs {
-    const float *sino;
-    float *img;
+    const void *sino;  // TI[n][nViews][nCols]
+    void *img;         // TO[n][nY][nX]
     const float4 *ray_geo;
     const int32_t *ray_n;
     const float2 *ray_inv;  // (1/du, 1/dw), 1e30 for a zero step
@@ -49,7 +50,7 @@ struct FpaArgs {
 // Per-pixel accumulation over all views.  kBorder: the workgroup's tile touches the
 // image edge, so the clamp window and the unbounded outer support of border pixels
 // apply; interior tiles skip them.
-template <int S, bool kBorder>
+template <int S, bool kBorder, typename TI, typename TO>
 __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, int s0)
 {
     const float fi = (float)i, fj = (float)j;
@@ -71,9 +72,10 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
         rho = sqrtf(ex * ex + ey * ey);
     }
 
-    const float *sino[S];
+    const TI *sino[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) sino[s] = A.sino + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+    for (int s = 0; s < S; ++s)
+        sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
 
     double acc[S];
 #pragma unroll
@@ -121,7 +123,7 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
                 }
             }
 #pragma unroll
-            for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sino[s][rbase + c], vsum[s]);
+            for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, load_f(sino[s], rbase + c), vsum[s]);
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s];
@@ -129,10 +131,11 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
 #pragma unroll
     for (int s = 0; s < S; ++s)
         if (s0 + s < A.nSlices)
-            A.img[(size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i] = (float)(acc[s] * (double)A.scale);
+            store_f(static_cast<TO *>(A.img), (size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i,
+                    (float)(acc[s] * (double)A.scale));
 }
 
-template <int S>
+template <int S, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
 {
     const int i = blockIdx.x * kAdjTx + threadIdx.x % kAdjTx;
@@ -142,15 +145,32 @@ __global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
     const bool border_tile = blockIdx.x == 0 || blockIdx.y == 0 || (int)(blockIdx.x + 1) * kAdjTx >= A.nX ||
                              (int)(blockIdx.y + 1) * kAdjTy >= A.nY;
     if (border_tile)
-        adjoint_pixel<S, true>(A, i, j, s0);
+        adjoint_pixel<S, true, TI, TO>(A, i, j, s0);
     else
-        adjoint_pixel<S, false>(A, i, j, s0);
+        adjoint_pixel<S, false, TI, TO>(A, i, j, s0);
 }
 
 }  // namespace
 
-fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img, int n,
-                              hipStream_t stream)
+template <int S>
+static fbct_status launch_fpa_t(fbct_dtype ti, fbct_dtype to, dim3 grid, hipStream_t stream, const FpaArgs &A)
+{
+    const dim3 block(kAdjTx * kAdjTy);
+    if (ti == FBCT_F32 && to == FBCT_F32)
+        hipLaunchKernelGGL((fp_adjoint_kernel<S, float, float>), grid, block, 0, stream, A);
+    else if (ti == FBCT_BF16 && to == FBCT_F32)
+        hipLaunchKernelGGL((fp_adjoint_kernel<S, bf16_t, float>), grid, block, 0, stream, A);
+    else if (ti == FBCT_F32 && to == FBCT_BF16)
+        hipLaunchKernelGGL((fp_adjoint_kernel<S, float, bf16_t>), grid, block, 0, stream, A);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16)
+        hipLaunchKernelGGL((fp_adjoint_kernel<S, bf16_t, bf16_t>), grid, block, 0, stream, A);
+    else
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp_adjoint: unknown storage type");
+    return FBCT_OK;
+}
+
+fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const void *sino, fbct_dtype ti,
+                              void *img, fbct_dtype to, int n, hipStream_t stream)
 {
     FpaArgs A;
     A.sino = sino;
@@ -175,19 +195,20 @@ fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     A.c_abs = fabsf(p.d.fpa_c_scale);
     A.far_x = p.d.fpa_far_x;
     A.far_y = p.d.fpa_far_y;
-    const dim3 block(kAdjTx * kAdjTy);
     const unsigned gx = (p.g.nX + kAdjTx - 1) / kAdjTx, gy = (p.g.nY + kAdjTy - 1) / kAdjTy;
     // Slices that share the geometry are processed together when there are enough
     // workgroups left to fill the device.
     const long long tiles = (long long)gx * gy;
+    fbct_status st;
     if (n >= 8 && tiles * ((n + 7) / 8) >= 1024)
-        hipLaunchKernelGGL(fp_adjoint_kernel<8>, dim3(gx, gy, (n + 7) / 8), block, 0, stream, A);
+        st = launch_fpa_t<8>(ti, to, dim3(gx, gy, (n + 7) / 8), stream, A);
     else if (n >= 4 && tiles * ((n + 3) / 4) >= 1024)
-        hipLaunchKernelGGL(fp_adjoint_kernel<4>, dim3(gx, gy, (n + 3) / 4), block, 0, stream, A);
+        st = launch_fpa_t<4>(ti, to, dim3(gx, gy, (n + 3) / 4), stream, A);
     else if (n >= 2 && tiles * ((n + 1) / 2) >= 1024)
-        hipLaunchKernelGGL(fp_adjoint_kernel<2>, dim3(gx, gy, (n + 1) / 2), block, 0, stream, A);
+        st = launch_fpa_t<2>(ti, to, dim3(gx, gy, (n + 1) / 2), stream, A);
     else
-        hipLaunchKernelGGL(fp_adjoint_kernel<1>, dim3(gx, gy, n), block, 0, stream, A);
+        st = launch_fpa_t<1>(ti, to, dim3(gx, gy, n), stream, A);
+    if (st != FBCT_OK) return st;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_adjoint_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

@@ -39,14 +39,19 @@ __all__ = [
 
 
 class FPFunction(torch.autograd.Function):
+    """FP with its matched adjoint as backward.  A bfloat16 image is read as BF16
+    storage (FP32 arithmetic); the sinogram is FP32 and the gradient takes the
+    image's dtype."""
+
     @staticmethod
     def forward(ctx, image, plan: FanBeamPlan):
         ctx.plan = plan
+        ctx.image_dtype = image.dtype
         return plan.fp(image)
 
     @staticmethod
     def backward(ctx, grad):
-        return ctx.plan.fp_adjoint(grad.contiguous()), None
+        return ctx.plan.fp_adjoint(grad.contiguous(), out_dtype=ctx.image_dtype), None
 
 
 class FPAdjointFunction(torch.autograd.Function):

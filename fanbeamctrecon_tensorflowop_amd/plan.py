@@ -35,6 +35,11 @@ def _as_angle_array(angle) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(angle, dtype=np.float32).reshape(-1))
 
 
+# fbct_dtype codes of the storage types the operators accept
+_DTYPE = {torch.float32: 0, torch.bfloat16: 1}
+_STORAGE = (torch.float32, torch.bfloat16)
+
+
 def _fptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
@@ -98,14 +103,14 @@ class FanBeamPlan:
         return h
 
     # ------------------------------------------------------------- execution
-    def _prep(self, x: torch.Tensor, shape, what: str) -> torch.Tensor:
+    def _prep(self, x: torch.Tensor, shape, what: str, dtypes=(torch.float32,)) -> torch.Tensor:
         if not isinstance(x, torch.Tensor):
             raise TypeError("%s: expected a torch.Tensor" % what)
         if not x.is_cuda:
             raise ValueError("%s: tensor must be on a HIP device (got %s); this library has no CPU path"
                              % (what, x.device))
-        if x.dtype != torch.float32:
-            raise TypeError("%s: expected float32, got %s" % (what, x.dtype))
+        if x.dtype not in dtypes:
+            raise TypeError("%s: expected %s, got %s" % (what, " or ".join(map(str, dtypes)), x.dtype))
         if x.dim() == 2:
             x = x.unsqueeze(0)
         if x.dim() != 3 or tuple(x.shape[1:]) != tuple(shape):
@@ -131,13 +136,34 @@ class FanBeamPlan:
         _lib.check(st, fn_name)
         return out
 
-    def fp(self, image: torch.Tensor) -> torch.Tensor:
-        """image [n, nY, nX] -> sinogram [n, nViews, nCols]."""
-        return self._run("fbct_fp", self._prep(image, self.image_shape, "fp"), self.sino_shape)
+    def _run_ex(self, fn_name: str, x: torch.Tensor, out_shape, out_dtype) -> torch.Tensor:
+        lib = _lib.load()
+        n = x.shape[0]
+        out = torch.empty((n,) + tuple(out_shape), dtype=out_dtype, device=x.device)
+        if n == 0:
+            return out
+        with torch.cuda.device(x.device):
+            stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+            st = getattr(lib, fn_name)(self._h, ctypes.c_void_p(x.data_ptr()), _DTYPE[x.dtype],
+                                       ctypes.c_void_p(out.data_ptr()), _DTYPE[out_dtype], int(n), stream)
+        _lib.check(st, fn_name)
+        return out
 
-    def fp_adjoint(self, sino: torch.Tensor) -> torch.Tensor:
-        """Exact transpose of :meth:`fp`: sinogram -> image."""
-        return self._run("fbct_fp_adjoint", self._prep(sino, self.sino_shape, "fp_adjoint"), self.image_shape)
+    def fp(self, image: torch.Tensor, out_dtype=torch.float32) -> torch.Tensor:
+        """image [n, nY, nX] -> sinogram [n, nViews, nCols].
+
+        float32 or bfloat16 storage for either side (arithmetic is FP32)."""
+        x = self._prep(image, self.image_shape, "fp", _STORAGE)
+        if x.dtype == torch.float32 and out_dtype == torch.float32:
+            return self._run("fbct_fp", x, self.sino_shape)
+        return self._run_ex("fbct_fp_ex", x, self.sino_shape, out_dtype)
+
+    def fp_adjoint(self, sino: torch.Tensor, out_dtype=torch.float32) -> torch.Tensor:
+        """Exact transpose of :meth:`fp`: sinogram -> image (float32 or bfloat16 storage)."""
+        y = self._prep(sino, self.sino_shape, "fp_adjoint", _STORAGE)
+        if y.dtype == torch.float32 and out_dtype == torch.float32:
+            return self._run("fbct_fp_adjoint", y, self.image_shape)
+        return self._run_ex("fbct_fp_adjoint_ex", y, self.image_shape, out_dtype)
 
     def bp(self, sino: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_bp", self._prep(sino, self.sino_shape, "bp"), self.image_shape)

@@ -124,10 +124,19 @@ fbct_status fbct_plan_upload(fbct_plan *plan);
  * Replaces project_fan_curve (FP/transform.cu:173-269; kernels :64-146). */
 fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slices, void *stream);
 
+/* fbct_fp with explicit storage types (fbct_dtype) for the image and the sinogram;
+ * BF16 is widened on load and rounded to nearest even on store, arithmetic is FP32. */
+fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void *sino, int sino_dtype,
+                       int n_slices, void *stream);
+
 /* Exact matched adjoint of fbct_fp (transpose of the same interpolation weights).
  * sino -> image.  No reference counterpart (the reference has no gradient). */
 fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, int n_slices,
                             void *stream);
+
+/* fbct_fp_adjoint with explicit storage types for the sinogram and the image. */
+fbct_status fbct_fp_adjoint_ex(fbct_plan *plan, const void *sino, int sino_dtype, void *image, int image_dtype,
+                               int n_slices, void *stream);
 
 /* Pixel-driven backprojection (unweighted).  sino -> image.
  * Replaces backproject_fan_curve of BP (BP/transform.cu:123-176; kernel :67-95). */

@@ -260,3 +260,49 @@ def test_fp_gradient_matches_oracle(small):
     err = rel_l2(g, g_ref)
     _say("fp grad rel_l2=%.3e" % err)
     assert err <= 1e-4  # residual y - Ax is formed in FP32 before the adjoint
+
+
+# -------------------------------------------------------------- BF16 storage
+
+def _bf16_round(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(torch.bfloat16).float().numpy()
+
+
+def test_fp_bf16_storage(clinical):
+    # BF16 image storage, FP32 arithmetic: exact against the oracle on the BF16-rounded
+    # image, and within 1e-3 of the FP32 image's projection
+    plan, o, img = clinical
+    xb = torch.from_numpy(img).cuda().to(torch.bfloat16)
+    got = host(plan.fp(xb))
+    e_storage = rel_l2(got, o.fp(_bf16_round(img)))
+    e_total = rel_l2(got, o.fp(img))
+    _say("fp bf16-in: vs rounded input %.3e, vs fp32 input %.3e" % (e_storage, e_total))
+    assert e_storage <= TOL and e_total <= 1e-3
+    out_b = plan.fp(torch.from_numpy(img).cuda(), out_dtype=torch.bfloat16)
+    assert out_b.dtype == torch.bfloat16
+    e_out = rel_l2(host(out_b.float()), o.fp(img))
+    _say("fp bf16-out: %.3e" % e_out)
+    assert e_out <= 4e-3  # one BF16 rounding of the output (2^-9 relative)
+
+
+def test_fp_adjoint_bf16_storage(clinical):
+    plan, o, img = clinical
+    y = (o.fp(img) * (1 + np.random.default_rng(17).normal(0, 0.01, size=(1, 984, 888)))).astype(np.float32)
+    got = host(plan.fp_adjoint(torch.from_numpy(y).cuda().to(torch.bfloat16)))
+    e_storage = rel_l2(got, o.fp_adjoint(_bf16_round(y)))
+    e_total = rel_l2(got, o.fp_adjoint(y))
+    _say("fp_adjoint bf16-in: vs rounded input %.3e, vs fp32 input %.3e" % (e_storage, e_total))
+    assert e_storage <= TOL and e_total <= 1e-3
+
+
+def test_bf16_image_gradient_dtype(small):
+    plan, o, img = small
+    x = torch.from_numpy(img).cuda().to(torch.bfloat16).requires_grad_(True)
+    y = ops.forward_project(x, plan)
+    assert y.dtype == torch.float32
+    (y ** 2).sum().backward()
+    assert x.grad.dtype == torch.bfloat16
+    g_ref = 2.0 * o.fp_adjoint(o.fp(_bf16_round(img)))
+    err = rel_l2(host(x.grad.float()), g_ref)
+    _say("bf16 grad rel_l2=%.3e" % err)
+    assert err <= 4e-3
