@@ -40,6 +40,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--slices", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="dispatch each operator once on the batch (plus a 256 MiB calibration copy) "
+                         "for counter collection under rocprofv3, then exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (slices shard across GPUs "
@@ -70,6 +73,22 @@ def main(argv=None):
     # the size of the signal, as between a prediction and a measurement)
     target = plan.fp(torch.roll(x, shifts=9, dims=2))
     torch.cuda.synchronize()
+
+    if args.profile_kernels:
+        # calibration: a plain device copy of 256 MiB (reads and writes 262144 KiB)
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        sino = target.contiguous()
+        for name, fn in (("fp", lambda: plan.fp(x)), ("fp_adjoint", lambda: plan.fp_adjoint(sino)),
+                         ("bp", lambda: plan.bp(sino)), ("fbp", lambda: plan.fbp(sino))):
+            fn()
+            torch.cuda.synchronize()
+            log("profile: %s dispatched" % name)
+        print(json.dumps({"profiled": ["copy 256MiB", "fp", "fp_adjoint", "bp", "fbp"], "slices": S,
+                          "fp_samples_per_slice": samples}), flush=True)
+        return
 
     # The step: FP of the batch, residual against a target, matched adjoint of the
     # residual.  BF16 storage: images and the residual sinogram are held in BF16 and

@@ -78,6 +78,7 @@ SIGNATURES = [
     ("fbct_fbp_backproject", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fbp_backproject_adjoint", _I, [_P, _P, _P, _I, _P]),
     ("fbct_ramp_filter", _I, [_P, _P, _P, ctypes.c_int64, _P]),
+    ("fbct_ramp_filter_ex", _I, [_P, _P, _P, ctypes.c_int64, _I, _P]),
     ("fbct_fbp_workspace_size", ctypes.c_size_t, [_P, _I]),
     ("fbct_fbp", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fbp_adjoint", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),

@@ -28,6 +28,13 @@ fbct_status check_dtype(int d, const char *what)
     return FBCT_OK;
 }
 
+fbct_status filter_rows(const fbct_plan &p, const fbct::DeviceTables &t, const float *in, float *out, int64_t rows,
+                        fbct_filter_mode mode, hipStream_t st)
+{
+    return mode == FBCT_FILTER_FP64 ? fbct::launch_ramp_filter(p, t, in, out, rows, st)
+                                    : fbct::launch_ramp_filter_mfma(p, t, in, out, rows, st);
+}
+
 size_t sino_bytes(const fbct_plan *p, int64_t n)
 {
     return static_cast<size_t>(n) * p->g.nViews * p->g.nCols * sizeof(float);
@@ -120,17 +127,29 @@ fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, fl
     return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, true, static_cast<hipStream_t>(stream));
 }
 
-fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows, void *stream)
+fbct_status fbct_ramp_filter_ex(fbct_plan *plan, const float *in, float *out, int64_t n_rows, int mode,
+                                void *stream)
 {
     if (!plan) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: null plan");
     if (n_rows < 0) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: negative row count");
+    if (mode != FBCT_FILTER_MFMA && mode != FBCT_FILTER_FP64)
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: unknown filter mode %d", mode);
     if (n_rows == 0) return FBCT_OK;
     if (!in || !out) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: null data pointer");
-    if (in == out) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: in and out must not overlap");
+    const size_t bytes = static_cast<size_t>(n_rows) * plan->g.nCols * sizeof(float);
+    const char *a = reinterpret_cast<const char *>(in), *b = reinterpret_cast<const char *>(out);
+    if (a < b + bytes && b < a + bytes)
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_ramp_filter: in and out must not overlap");
     const fbct::DeviceTables *t = nullptr;
     fbct_status s;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::launch_ramp_filter(*plan, *t, in, out, n_rows, static_cast<hipStream_t>(stream));
+    return filter_rows(*plan, *t, in, out, n_rows, static_cast<fbct_filter_mode>(mode),
+                       static_cast<hipStream_t>(stream));
+}
+
+fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows, void *stream)
+{
+    return fbct_ramp_filter_ex(plan, in, out, n_rows, FBCT_FILTER_MFMA, stream);
 }
 
 size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices)
@@ -151,7 +170,7 @@ fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_sli
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float *q = static_cast<float *>(workspace);
-    if ((s = fbct::launch_ramp_filter(*plan, *t, sino, q, (int64_t)n_slices * plan->g.nViews, st)) != FBCT_OK)
+    if ((s = filter_rows(*plan, *t, sino, q, (int64_t)n_slices * plan->g.nViews, FBCT_FILTER_MFMA, st)) != FBCT_OK)
         return s;
     return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);
 }
@@ -170,7 +189,7 @@ fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, i
     float *q = static_cast<float *>(workspace);
     if ((s = fbct::launch_bp_adjoint(*plan, *t, image, q, n_slices, true, st)) != FBCT_OK) return s;
     // The centre-cropped convolution matrix H[c][m] = h[c-m] is symmetric (h[-n] = h[n]).
-    return fbct::launch_ramp_filter(*plan, *t, q, sino, (int64_t)n_slices * plan->g.nViews, st);
+    return filter_rows(*plan, *t, q, sino, (int64_t)n_slices * plan->g.nViews, FBCT_FILTER_MFMA, st);
 }
 
 fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices, void *workspace,

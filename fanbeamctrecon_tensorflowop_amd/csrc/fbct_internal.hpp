@@ -14,9 +14,16 @@ namespace fbct {
 
 // FP ray-march tiling (see fp.hip): rays per workgroup bundle and slab width in pixels.
 constexpr int kFpThreads = 256;
-constexpr int kFpLanesPerRay = 4;
-constexpr int kFpRays = kFpThreads / kFpLanesPerRay;
 constexpr int kFpSlab = 32;
+// Bundle configurations: rays per workgroup (lanes per ray = kFpThreads / rays).
+constexpr int kFpNumCfg = 2;
+constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
+
+struct FpBundleCfg {
+    int rays;  // rays per bundle
+    int nb;    // bundles per view
+    int cap;   // LDS rows reserved for the minor extent of a slab
+};
 
 // Device-resident copy of a plan's tables on one HIP device.
 struct DeviceTables {
@@ -24,10 +31,11 @@ struct DeviceTables {
     int32_t *ray_n = nullptr;      // [nViews*nCols]
     float4 *ray_geo = nullptr;     // [nViews*nCols] (u0, w0, du, dw)
     float2 *ray_inv = nullptr;     // [nViews*nCols] (1/du, 1/dw)
-    uint8_t *fp_major = nullptr;   // [nViews*nFpBundles] 1 = x-major
+    uint8_t *fp_major[kFpNumCfg] = {};  // per bundle config: [nViews*nb] 1 = x-major
     float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
     double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
     double *ramp = nullptr;        // [2*nCols-1]
+    uint16_t *ramp_planes = nullptr;  // [3][Np][Kp] BF16 split planes of the Toeplitz matrix H
 };
 
 // Scalars the kernels need, derived once in FP64 on the host.
@@ -35,8 +43,7 @@ struct Derived {
     float fp_scale;       // 0.125 * dx
     float two_r;          // 2 * rFOV/dx (FP sample coordinate extent)
     int fp_nslab;         // slabs along the major axis
-    int fp_cap;           // LDS rows reserved for the minor extent of a slab
-    int fp_nbundles;      // ceil(nCols / kFpRays)
+    FpBundleCfg fpb[kFpNumCfg];
     float bp_xi_scale;    // 1/dCol
     float bp_xi_offset;   // dShift - dLeft/dCol
     float r2;             // rFOV*rFOV (float, as the reference)
@@ -49,6 +56,7 @@ struct Derived {
     float fpa_c_scale;    // 1/dCol
     float fpa_c_off;      // -dLeft/dCol - 1/2
     float fpa_far_x, fpa_far_y;  // 2r - n + 1/2: sample reach beyond the last pixel centre
+    int ramp_np, ramp_kp;         // padded columns / depth of the MFMA filter operand
 };
 
 }  // namespace fbct
@@ -61,10 +69,11 @@ struct fbct_plan {
     std::vector<int32_t> ray_n;
     std::vector<float> ray_geo;     // 4 per ray
     std::vector<float> ray_inv;     // 2 per ray
-    std::vector<uint8_t> fp_major;
+    std::vector<uint8_t> fp_major[fbct::kFpNumCfg];
     std::vector<float> view_cs;     // 2 per view
     std::vector<double> view_csd;   // 2 per view
     std::vector<double> ramp;       // 2*nCols-1
+    std::vector<uint16_t> ramp_planes;  // BF16 hi/lo/ll planes of H, [3][Np][Kp]
     int64_t fp_samples = 0;
     fbct::Derived d{};
 
@@ -92,5 +101,7 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
                               float *sino, int n, bool weighted, hipStream_t stream);
 fbct_status launch_ramp_filter(const fbct_plan &p, const DeviceTables &t, const float *in,
                                float *out, int64_t rows, hipStream_t stream);
+fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in,
+                                    float *out, int64_t rows, hipStream_t stream);
 
 }  // namespace fbct

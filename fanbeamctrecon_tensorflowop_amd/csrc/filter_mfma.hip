@@ -1,0 +1,142 @@
+// filter_mfma.hip -- ramp filter as a Toeplitz GEMM on the gfx950 BF16 matrix cores.
+//
+// The centre-cropped convolution of FBP/transform.cu:163-174 is the product
+//     Q[rows][nC] = P[rows][nC] * H[nC][nC],   H[m][c] = h[c - m],
+// with H symmetric (h[-n] = h[n]).  BF16 keeps the FP32 exponent range but only 8
+// mantissa bits, so both operands are split three ways, x = x_hi + x_lo + x_ll (each
+// a BF16 number), and six products are accumulated in FP32 by
+// v_mfma_f32_16x16x32_bf16:
+//     hi*hi + hi*lo + lo*hi + lo*lo + hi*ll + ll*hi,
+// which carries the same information as an FP32 product (the dropped terms are below
+// 2^-24 relative).  H's three planes are precomputed per plan (padded with zeros to
+// [Np][Kp]); P is split in registers as it is loaded.
+//
+// Tiling: a wave computes 16 rows x 64 columns (4 MFMA tiles) over K in steps of 32;
+// a workgroup of 4 waves covers 64 rows.  Operand fragments are loaded straight from
+// global memory (P rows as FP32, H rows as BF16: H is symmetric, so the B fragment of
+// column c is row c of H, contiguous in k).
+// Lane layout of v_mfma_f32_16x16x32_bf16: A lane l holds row l%16, k = 8*(l/16)+j;
+// B lane l holds column l%16, k = 8*(l/16)+j; D lane l holds rows 4*(l/16)+j, column l%16.
+
+#include <hip/hip_runtime.h>
+
+#include "fbct_internal.hpp"
+
+namespace fbct {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaK = 32;
+constexpr int kWaveCols = 64;  // 4 tiles of 16
+
+struct MfmaArgs {
+    const float *in;
+    float *out;
+    const __bf16 *hp;  // [3][Np][Kp] split planes of H
+    long long rows;
+    int nC, Np, Kp;
+};
+
+__device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &ll)
+{
+    hi = (__bf16)x;
+    const float r1 = x - (float)hi;
+    lo = (__bf16)r1;
+    const float r2 = r1 - (float)lo;
+    ll = (__bf16)r2;
+}
+
+// kAligned: every row starts 16-byte aligned (nC % 4 == 0, aligned base), so the 8
+// k-values of an A fragment load as two float4.
+template <bool kAligned>
+__global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const long long r0 = (long long)blockIdx.x * 64 + wave * 16;
+    const int c0 = blockIdx.y * kWaveCols;
+    const long long arow = r0 + (lane & 15);
+    const bool arow_ok = arow < A.rows;
+    const float *prow = A.in + (arow_ok ? arow : 0) * (long long)A.nC;
+    const int kl = 8 * (lane >> 4);  // this lane's k offset within a 32-wide step
+    const size_t plane = (size_t)A.Np * A.Kp;
+
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int k0 = 0; k0 < A.Kp; k0 += kMfmaK) {
+        // A fragment: 8 consecutive k of one row, split into three BF16 planes
+        float x[8];
+        const int kb = k0 + kl;
+        if (kAligned && arow_ok && kb + 8 <= A.nC) {
+            const float4 u = *reinterpret_cast<const float4 *>(prow + kb);
+            const float4 w = *reinterpret_cast<const float4 *>(prow + kb + 4);
+            x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = w.x, x[5] = w.y, x[6] = w.z, x[7] = w.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = (arow_ok && kb + j < A.nC) ? prow[kb + j] : 0.0f;
+        }
+        bf16x8 ah, al, all;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            __bf16 h, l, m;
+            split3(x[j], h, l, m);
+            ah[j] = h, al[j] = l, all[j] = m;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int col = c0 + 16 * t + (lane & 15);  // < Np: planes are zero-padded
+            const size_t off = (size_t)col * A.Kp + kb;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(A.hp + off);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(A.hp + plane + off);
+            const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(A.hp + 2 * plane + off);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(all, bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bll, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int col = c0 + 16 * t + (lane & 15);
+        if (col >= A.nC) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long row = r0 + 4 * (lane >> 4) + j;
+            if (row < A.rows) A.out[row * A.nC + col] = acc[t][j];
+        }
+    }
+}
+
+}  // namespace
+
+fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in, float *out,
+                                    int64_t rows, hipStream_t stream)
+{
+    MfmaArgs A;
+    A.in = in;
+    A.out = out;
+    A.hp = reinterpret_cast<const __bf16 *>(t.ramp_planes);
+    A.rows = rows;
+    A.nC = p.g.nCols;
+    A.Np = p.d.ramp_np;
+    A.Kp = p.d.ramp_kp;
+    const long long gx = (rows + 63) / 64;
+    if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
+    dim3 grid((unsigned)gx, (p.g.nCols + kWaveCols - 1) / kWaveCols);
+    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 && p.g.nCols % 4 == 0;
+    if (aligned)
+        hipLaunchKernelGGL(ramp_filter_mfma_kernel<true>, grid, dim3(256), 0, stream, A);
+    else
+        hipLaunchKernelGGL(ramp_filter_mfma_kernel<false>, grid, dim3(256), 0, stream, A);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_filter_mfma_kernel launch: %s", hipGetErrorString(e));
+    return FBCT_OK;
+}
+
+}  // namespace fbct

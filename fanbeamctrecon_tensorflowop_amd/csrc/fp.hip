@@ -6,12 +6,12 @@
 //
 // Design here:
 //   * per-ray (u0, w0, du, dw, N) come from the plan's FP64 table;
-//   * a workgroup owns a bundle of kFpRays adjacent rays of one view and marches them
+//   * a workgroup owns a bundle of 256/G adjacent rays of one view and marches them
 //     slab by slab along the axis the bundle is closest to;  for every slab it stages
 //     the exact bounding box of the taps its samples touch into LDS (clamped to the
 //     image edge, which reproduces the clamp-to-edge texture address mode) and
 //     interpolates from LDS with exact FP32 bilinear weights;
-//   * kFpLanesPerRay lanes share a ray, each keeps its own FP32 partial sum across all
+//   * G lanes share a ray, each keeps its own FP32 partial sum across all
 //     slabs; one cross-lane reduction per ray at the end and one coalesced store.
 // A sample belongs to the slab containing its major coordinate, decided from the same
 // FP32 position the interpolation uses, so every sample is counted exactly once.
@@ -96,7 +96,7 @@ __device__ __forceinline__ int wave_max(int v)
 // S slices that share the geometry are projected together: the LDS tile interleaves
 // them ([minor][major][slice]) so one sample position, its weights and its LDS
 // address serve S slices, with S-wide LDS reads.
-template <int S, typename TI, typename TO>
+template <int S, int G, typename TI, typename TO>
 __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 {
     using TapT = typename Taps<S>::T;
@@ -107,9 +107,10 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
         reinterpret_cast<int(*)[2][kFpThreads / 64]>(tile + (size_t)kPitch * A.cap * S);
 
     const int tid = threadIdx.x;
-    const int lane = tid % kFpLanesPerRay;
+    constexpr int kRays = kFpThreads / G;
+    const int lane = tid % G;
     const int b = blockIdx.x, v = blockIdx.y, s0 = blockIdx.z * S;
-    const int c = b * kFpRays + tid / kFpLanesPerRay;
+    const int c = b * kRays + tid / G;
     const bool xmajor = A.major[(size_t)v * A.nb + b] != 0;
 
     float a0 = 0.0f, m0 = 0.0f, da = 0.0f, dm = 0.0f;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
         float acc[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] = 0.0f;
-        for (float kf = (float)(ka + lane); kf < kend; kf += (float)kFpLanesPerRay) {
+        for (float kf = (float)(ka + lane); kf < kend; kf += (float)G) {
             const float a = fmaf(kf, da, a0);
             const float m = fmaf(kf, dm, m0);
             const float fa = floorf(a), fm = floorf(m);
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
 #pragma unroll
-        for (int off = kFpLanesPerRay / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
+        for (int off = G / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
         if (lane == 0 && c < A.nCols && s0 + s < A.nSlices)
             store_f(static_cast<TO *>(A.sino), (size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c,
                     (float)(accd[s] * (double)A.scale));
@@ -249,20 +250,20 @@ __device__ int g_fp_err = 0;
 
 }  // namespace
 
-template <int S, typename TI, typename TO>
+template <int S, int G, typename TI, typename TO>
 static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, int *err)
 {
-    hipLaunchKernelGGL((fp_kernel<S, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, err);
+    hipLaunchKernelGGL((fp_kernel<S, G, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, err);
 }
 
-template <int S>
+template <int S, int G>
 static fbct_status launch_fp_t(fbct_dtype ti, fbct_dtype to, dim3 grid, size_t lds, hipStream_t stream,
                                const FpArgs &A, int *err)
 {
-    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, float, float>(grid, lds, stream, A, err);
-    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, bf16_t, float>(grid, lds, stream, A, err);
-    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, float, bf16_t>(grid, lds, stream, A, err);
-    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, bf16_t, bf16_t>(grid, lds, stream, A, err);
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, G, float, float>(grid, lds, stream, A, err);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, G, bf16_t, float>(grid, lds, stream, A, err);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, G, float, bf16_t>(grid, lds, stream, A, err);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, G, bf16_t, bf16_t>(grid, lds, stream, A, err);
     else return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp: unknown storage type");
     return FBCT_OK;
 }
@@ -275,36 +276,53 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.sino = sino;
     A.ray_geo = t.ray_geo;
     A.ray_n = t.ray_n;
-    A.major = t.fp_major;
     A.img_stride = (long long)p.g.nX * p.g.nY;
     A.sino_stride = (long long)p.g.nViews * p.g.nCols;
     A.nX = p.g.nX;
     A.nY = p.g.nY;
     A.nViews = p.g.nViews;
     A.nCols = p.g.nCols;
-    A.nb = p.d.fp_nbundles;
     A.nslab = p.d.fp_nslab;
-    A.cap = p.d.fp_cap;
     A.scale = p.d.fp_scale;
     A.nSlices = n;
     int *err = nullptr;
     hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&err), HIP_SYMBOL(g_fp_err));
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetSymbolAddress: %s", hipGetErrorString(e));
-    // Slices per workgroup: share each sample's position and weights across up to 4
-    // slices when the grid still holds enough workgroups and the tile fits in LDS.
-    const size_t lds1 = (size_t)kPitch * p.d.fp_cap * sizeof(float);
+
+    // Slices per workgroup S and bundle configuration: share each sample's position and
+    // weights across 4 slices with 32-ray bundles (smaller tile), else 2 or 1 slices with
+    // 64-ray bundles, while the grid keeps >= 2048 workgroups and the tile <= 80 KB.
     const size_t red_bytes = 2 * 2 * (kFpThreads / 64) * sizeof(int);
-    const long long wg1 = (long long)p.d.fp_nbundles * p.g.nViews;
-    int S = 1;
-    if (n >= 4 && lds1 * 4 <= 64 * 1024 && wg1 * ((n + 3) / 4) >= 2048) S = 4;
-    else if (n >= 2 && lds1 * 2 <= 64 * 1024 && wg1 * ((n + 1) / 2) >= 2048) S = 2;
-    if (lds1 * S + red_bytes > 160 * 1024)
-        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds1 * S);
-    dim3 grid(p.d.fp_nbundles, p.g.nViews, (n + S - 1) / S);
+    const size_t kMaxTile = 80 * 1024;
+    auto tile_bytes = [&](int cfg, int S) { return (size_t)kPitch * p.d.fpb[cfg].cap * sizeof(float) * S; };
+    auto grid_wgs = [&](int cfg, int S) { return (long long)p.d.fpb[cfg].nb * p.g.nViews * ((n + S - 1) / S); };
+    int S = 1, cfg = 0;
+    if (n >= 4 && tile_bytes(1, 4) <= kMaxTile && grid_wgs(1, 4) >= 2048) {
+        S = 4;
+        cfg = 1;
+    } else if (n >= 2 && grid_wgs(0, 2) >= 2048) {
+        S = 2;
+        cfg = tile_bytes(0, 2) <= kMaxTile ? 0 : 1;
+    } else if (tile_bytes(0, 1) > kMaxTile) {
+        cfg = 1;
+    }
+    const FpBundleCfg &bc = p.d.fpb[cfg];
+    A.major = t.fp_major[cfg];
+    A.nb = bc.nb;
+    A.cap = bc.cap;
+    const size_t lds = tile_bytes(cfg, S) + red_bytes;
+    if (lds > 160 * 1024)
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds);
+    dim3 grid(bc.nb, p.g.nViews, (n + S - 1) / S);
     fbct_status st;
-    if (S == 4) st = launch_fp_t<4>(ti, to, grid, lds1 * 4 + red_bytes, stream, A, err);
-    else if (S == 2) st = launch_fp_t<2>(ti, to, grid, lds1 * 2 + red_bytes, stream, A, err);
-    else st = launch_fp_t<1>(ti, to, grid, lds1 + red_bytes, stream, A, err);
+    if (cfg == 0) {
+        if (S == 2) st = launch_fp_t<2, 4>(ti, to, grid, lds, stream, A, err);
+        else st = launch_fp_t<1, 4>(ti, to, grid, lds, stream, A, err);
+    } else {
+        if (S == 4) st = launch_fp_t<4, 8>(ti, to, grid, lds, stream, A, err);
+        else if (S == 2) st = launch_fp_t<2, 8>(ti, to, grid, lds, stream, A, err);
+        else st = launch_fp_t<1, 8>(ti, to, grid, lds, stream, A, err);
+    }
     if (st != FBCT_OK) return st;
     e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));

@@ -139,6 +139,23 @@ static RayGeo ray_geometry(const fbct_fan_geom &g, double beta, int c)
     return out;
 }
 
+// Round a float to the nearest-even bfloat16 (raw bits) and back.
+static uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return static_cast<uint16_t>((u >> 16) | 0x40u);
+    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+static double bf16_value(uint16_t b)
+{
+    const uint32_t u = static_cast<uint32_t>(b) << 16;
+    float f;
+    std::memcpy(&f, &u, sizeof(f));
+    return static_cast<double>(f);
+}
+
 static void build_tables(fbct_plan &p)
 {
     const fbct_fan_geom &g = p.g;
@@ -175,48 +192,54 @@ static void build_tables(fbct_plan &p)
     }
     p.fp_samples = samples;
 
-    // FP bundles: rays [b*kFpRays, (b+1)*kFpRays) of a view march slab by slab along
-    // the axis their central ray is closest to.  fp_cap bounds the minor extent of the
-    // taps of one slab, from the ray lines in FP64 plus a margin that covers the FP32
-    // sample positions; the kernel derives the exact extent from its own positions.
+    // FP bundles: rays [b*rays, (b+1)*rays) of a view march slab by slab along the
+    // axis their central ray is closest to.  cap bounds the minor extent of the taps
+    // of one slab, from the ray lines in FP64 plus a margin that covers the FP32 sample
+    // positions; the kernel derives the exact extent from its own positions.
     Derived &d = p.d;
     d.fp_scale = static_cast<float>(0.125 * static_cast<double>(g.dx));
     d.two_r = static_cast<float>(2.0 * r);
     d.fp_nslab = static_cast<int>(std::floor((2.0 * r + 1.0) / kFpSlab)) + 1;
-    d.fp_nbundles = (nC + kFpRays - 1) / kFpRays;
-    p.fp_major.assign(static_cast<size_t>(nV) * d.fp_nbundles, 1);
-    double cap = 0.0;
-    for (int v = 0; v < nV; ++v) {
-        const double beta = static_cast<double>(p.angles[v]);
-        const double su = Rp * std::cos(beta) + r - 0.5;  // source, texel coordinates
-        const double sw = Rp * std::sin(beta) + r - 0.5;
-        for (int b = 0; b < d.fp_nbundles; ++b) {
-            const int c0 = b * kFpRays;
-            const int c1 = std::min(nC, c0 + kFpRays) - 1;
-            const double gmid = static_cast<double>(g.dLeft) + (0.5 * (c0 + c1) + 0.5) * static_cast<double>(g.dCol);
-            const double thm = beta + gmid;
-            const bool xmajor = std::fabs(std::cos(thm)) >= std::fabs(std::sin(thm));
-            p.fp_major[static_cast<size_t>(v) * d.fp_nbundles + b] = xmajor ? 1 : 0;
-            const double sa = xmajor ? su : sw, sm = xmajor ? sw : su;
-            double slope_max = 0.0, lo_a = 1e300, hi_a = -1e300, lo_b = 1e300, hi_b = -1e300;
-            for (int c : {c0, c1}) {
-                const double th = beta + static_cast<double>(g.dLeft) + (c + 0.5) * static_cast<double>(g.dCol);
-                const double da = xmajor ? std::cos(th) : std::sin(th);
-                const double dm = xmajor ? std::sin(th) : std::cos(th);
-                const double slope = dm / da;
-                slope_max = std::max(slope_max, std::fabs(slope));
-                const double ma = sm + (-1.0 - sa) * slope;
-                const double mb = sm + (2.0 * r + 1.0 - sa) * slope;
-                lo_a = std::min(lo_a, ma);
-                hi_a = std::max(hi_a, ma);
-                lo_b = std::min(lo_b, mb);
-                hi_b = std::max(hi_b, mb);
+    for (int cfg = 0; cfg < kFpNumCfg; ++cfg) {
+        const int rays = kFpCfgRays[cfg];
+        FpBundleCfg &bc = d.fpb[cfg];
+        bc.rays = rays;
+        bc.nb = (nC + rays - 1) / rays;
+        std::vector<uint8_t> &major = p.fp_major[cfg];
+        major.assign(static_cast<size_t>(nV) * bc.nb, 1);
+        double cap = 0.0;
+        for (int v = 0; v < nV; ++v) {
+            const double beta = static_cast<double>(p.angles[v]);
+            const double su = Rp * std::cos(beta) + r - 0.5;  // source, texel coordinates
+            const double sw = Rp * std::sin(beta) + r - 0.5;
+            for (int b = 0; b < bc.nb; ++b) {
+                const int c0 = b * rays;
+                const int c1 = std::min(nC, c0 + rays) - 1;
+                const double gmid = static_cast<double>(g.dLeft) + (0.5 * (c0 + c1) + 0.5) * static_cast<double>(g.dCol);
+                const double thm = beta + gmid;
+                const bool xmajor = std::fabs(std::cos(thm)) >= std::fabs(std::sin(thm));
+                major[static_cast<size_t>(v) * bc.nb + b] = xmajor ? 1 : 0;
+                const double sa = xmajor ? su : sw, sm = xmajor ? sw : su;
+                double slope_max = 0.0, lo_a = 1e300, hi_a = -1e300, lo_b = 1e300, hi_b = -1e300;
+                for (int c : {c0, c1}) {
+                    const double th = beta + static_cast<double>(g.dLeft) + (c + 0.5) * static_cast<double>(g.dCol);
+                    const double da = xmajor ? std::cos(th) : std::sin(th);
+                    const double dm = xmajor ? std::sin(th) : std::cos(th);
+                    const double slope = dm / da;
+                    slope_max = std::max(slope_max, std::fabs(slope));
+                    const double ma = sm + (-1.0 - sa) * slope;
+                    const double mb = sm + (2.0 * r + 1.0 - sa) * slope;
+                    lo_a = std::min(lo_a, ma);
+                    hi_a = std::max(hi_a, ma);
+                    lo_b = std::min(lo_b, mb);
+                    hi_b = std::max(hi_b, mb);
+                }
+                const double spread = std::max(hi_a - lo_a, hi_b - lo_b);
+                cap = std::max(cap, spread + kFpSlab * slope_max + 4.0);
             }
-            const double spread = std::max(hi_a - lo_a, hi_b - lo_b);
-            cap = std::max(cap, spread + kFpSlab * slope_max + 4.0);
         }
+        bc.cap = static_cast<int>(std::ceil(std::min(cap, std::ceil(2.0 * r) + 3.0)));
     }
-    d.fp_cap = static_cast<int>(std::ceil(std::min(cap, std::ceil(2.0 * r) + 3.0)));
 
     // BP: xi_raw = (atan(t/U) - dLeft)/dCol + dShift  (BP/transform.cu:85).
     d.bp_xi_scale = static_cast<float>(1.0 / static_cast<double>(g.dCol));
@@ -239,6 +262,29 @@ static void build_tables(fbct_plan &p)
             h = 0.0;
         }
         p.ramp[n + nC - 1] = h * kappa;
+    }
+
+    // MFMA filter operand: H[m][c] = h[c - m] (symmetric), split into three BF16 planes
+    // hi + lo + ll (each the BF16 rounding of what the previous planes left), padded with
+    // zeros to Np = 64-multiple rows and Kp = 32-multiple columns.
+    d.ramp_np = (nC + 63) / 64 * 64;
+    d.ramp_kp = (nC + 31) / 32 * 32;
+    p.ramp_planes.assign(static_cast<size_t>(3) * d.ramp_np * d.ramp_kp, 0);
+    {
+        const size_t plane = static_cast<size_t>(d.ramp_np) * d.ramp_kp;
+        for (int c = 0; c < nC; ++c)
+            for (int m = 0; m < nC; ++m) {
+                const double x = p.ramp[c - m + nC - 1];
+                const uint16_t hi = bf16_rne(static_cast<float>(x));
+                const double r1 = x - bf16_value(hi);
+                const uint16_t lo = bf16_rne(static_cast<float>(r1));
+                const double r2 = r1 - bf16_value(lo);
+                const uint16_t ll = bf16_rne(static_cast<float>(r2));
+                const size_t o = static_cast<size_t>(c) * d.ramp_kp + m;
+                p.ramp_planes[o] = hi;
+                p.ramp_planes[plane + o] = lo;
+                p.ramp_planes[2 * plane + o] = ll;
+            }
     }
 
     // BP in-range test 0 < xi < nCols <=> g in (dLeft - dShift dCol, dLeft + (nCols - dShift) dCol)
@@ -282,11 +328,12 @@ static void free_tables(DeviceTables &t)
     if (t.device >= 0) (void)hipSetDevice(t.device);
     (void)hipFree(t.ray_n);
     (void)hipFree(t.ray_geo);
-    (void)hipFree(t.fp_major);
+    for (int cfg = 0; cfg < kFpNumCfg; ++cfg) (void)hipFree(t.fp_major[cfg]);
     (void)hipFree(t.ray_inv);
     (void)hipFree(t.view_cs);
     (void)hipFree(t.view_csd);
     (void)hipFree(t.ramp);
+    (void)hipFree(t.ramp_planes);
     if (cur >= 0) (void)hipSetDevice(cur);
     t = DeviceTables{};
 }
@@ -307,11 +354,13 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
     fbct_status s;
     if ((s = upload(&t.ray_n, plan->ray_n.data(), plan->ray_n.size() * sizeof(int32_t))) != FBCT_OK ||
         (s = upload(&t.ray_geo, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float))) != FBCT_OK ||
-        (s = upload(&t.fp_major, plan->fp_major.data(), plan->fp_major.size())) != FBCT_OK ||
+        (s = upload(&t.fp_major[0], plan->fp_major[0].data(), plan->fp_major[0].size())) != FBCT_OK ||
+        (s = upload(&t.fp_major[1], plan->fp_major[1].data(), plan->fp_major[1].size())) != FBCT_OK ||
         (s = upload(&t.ray_inv, plan->ray_inv.data(), plan->ray_inv.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_csd, plan->view_csd.data(), plan->view_csd.size() * sizeof(double))) != FBCT_OK ||
-        (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK) {
+        (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK ||
+        (s = upload(&t.ramp_planes, plan->ramp_planes.data(), plan->ramp_planes.size() * sizeof(uint16_t))) != FBCT_OK) {
         std::string msg = g_last_error;
         free_tables(t);
         g_last_error = msg;

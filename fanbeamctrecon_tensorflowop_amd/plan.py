@@ -190,7 +190,11 @@ class FanBeamPlan:
     def fdk(self, proj: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_fdk", self._prep(proj, self.sino_shape, "fdk"), self.image_shape, extra_ws=True)
 
-    def ramp_filter(self, sino: torch.Tensor) -> torch.Tensor:
+    def ramp_filter(self, sino: torch.Tensor, mode: str = "mfma") -> torch.Tensor:
+        """Ramp filter of every detector row.  mode "mfma" (BF16 matrix cores, three-way
+        split operands) or "fp64" (direct convolution with FP64 accumulation)."""
+        if mode not in ("mfma", "fp64"):
+            raise ValueError("ramp_filter: mode must be 'mfma' or 'fp64'")
         x = self._prep(sino, self.sino_shape, "ramp_filter")
         lib = _lib.load()
         out = torch.empty_like(x)
@@ -198,8 +202,9 @@ class FanBeamPlan:
             return out
         with torch.cuda.device(x.device):
             stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
-            st = lib.fbct_ramp_filter(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                      ctypes.c_int64(x.shape[0] * self.n_views), stream)
+            st = lib.fbct_ramp_filter_ex(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                         ctypes.c_int64(x.shape[0] * self.n_views), 1 if mode == "fp64" else 0,
+                                         stream)
         _lib.check(st, "fbct_ramp_filter")
         return out
 

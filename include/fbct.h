@@ -56,6 +56,14 @@ typedef enum fbct_dtype {
     FBCT_BF16 = 1
 } fbct_dtype;
 
+/* Arithmetic of the ramp filter.  MFMA: Toeplitz GEMM on the BF16 matrix cores with
+ * both operands split three ways (six products, FP32 accumulation; FP32-level accuracy).
+ * FP64: direct convolution on the vector ALUs with FP64 products and sums. */
+typedef enum fbct_filter_mode {
+    FBCT_FILTER_MFMA = 0,
+    FBCT_FILTER_FP64 = 1
+} fbct_filter_mode;
+
 /* Geometry decoded from the reference's 16-float `parameter` attribute exactly as
  * the op constructors decode it (FP/transform.cc:59-76, FDK/FBP.cc:61-86):
  * float arithmetic, PI = 3.14159265359f, angles converted from degrees. */
@@ -147,11 +155,15 @@ fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, in
                             void *stream);
 
 /* Ramp filter of every row: out[r][c] = sum_m in[r][m] h[c-m] (centre-cropped full
- * linear convolution; FP64 accumulation).  Replaces the host IPP filter
- * (FBP/transform.cu:147-176, FDK/GPURecon_GPU.cu:184-217).  in and out must not
- * overlap. */
+ * linear convolution) on the matrix cores (FBCT_FILTER_MFMA).  Replaces the host IPP
+ * filter (FBP/transform.cu:147-176, FDK/GPURecon_GPU.cu:184-217).  in and out must
+ * not overlap. */
 fbct_status fbct_ramp_filter(fbct_plan *plan, const float *in, float *out, int64_t n_rows,
                              void *stream);
+
+/* fbct_ramp_filter with an explicit fbct_filter_mode. */
+fbct_status fbct_ramp_filter_ex(fbct_plan *plan, const float *in, float *out, int64_t n_rows, int mode,
+                                void *stream);
 
 /* Workspace in bytes needed by fbct_fbp / fbct_fbp_adjoint / fbct_fdk for n_slices. */
 size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices);

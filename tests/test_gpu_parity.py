@@ -90,13 +90,21 @@ def test_fbp_matches_oracle(which, request):
     assert err <= TOL
 
 
-def test_ramp_filter_matches_oracle(small):
-    plan, o, img = small
-    sino = np.random.default_rng(11).normal(size=(2, 360, 512)).astype(np.float32)
-    got = host(plan.ramp_filter(dev(sino)))
-    err = rel_l2(got, o.ramp_filter(sino))
-    _say("ramp rel_l2=%.3e" % err)
-    assert err <= 1e-6
+@pytest.mark.parametrize("mode,tol", [("mfma", 1e-5), ("fp64", 1e-6)])
+@pytest.mark.parametrize("which", ["small", "clinical"])
+def test_ramp_filter_matches_oracle(which, mode, tol, request):
+    # mfma: BF16 matrix cores with three-way split operands and FP32 accumulation
+    # (FP32-GEMM-level error); fp64: FP64 products and sums (only the output rounding)
+    plan, o, img = request.getfixturevalue(which)
+    for kind in ("sinogram", "noise"):
+        if kind == "sinogram":
+            sino = o.fp(img).astype(np.float32)
+        else:
+            sino = np.random.default_rng(11).normal(size=(2,) + plan.sino_shape).astype(np.float32)
+        got = host(plan.ramp_filter(dev(sino), mode=mode))
+        err = rel_l2(got, o.ramp_filter(sino))
+        _say("ramp %s %s %s rel_l2=%.3e" % (mode, which, kind, err))
+        assert err <= tol
 
 
 def test_fdk_equals_fbp(small):
