@@ -5,10 +5,17 @@
 // FBP/transform.cu:61-89 (times 1/(U^2+t^2)), launched once per view with a memcpy
 // and a device sync per view (BP/transform.cu:110-118).
 //
-// Here one launch covers every view and slice: a thread owns a pixel, loops over all
-// views with the per-view (cos, sin) read as uniform scalars, accumulates in a
-// register and stores once.  The detector row is read with two gathers per update;
-// the 1-D texture's clamp-to-edge linear filter is reproduced with exact FP32 weights.
+// Here one launch covers every view and slice: a thread owns a pixel of S slices, loops
+// over all views with the per-view (cos, sin) read as uniform scalars, computes the
+// column coordinate, in-range decision, taps and weight once per (pixel, view) and
+// applies them to the S slices, accumulates in registers and stores once.  The detector
+// row is read with two gathers per update; the 1-D texture's clamp-to-edge linear
+// filter is reproduced with exact FP32 weights.
+//
+// atan(t/U) is evaluated as z + z^3 R(z^2) with a correction polynomial R fitted per
+// plan over the range FOV pixels reach, |t/U| <= rFOV/sqrt(R^2 - rFOV^2) (relative error
+// below 1e-8 before FP32 evaluation, about one ulp after); plans whose range needs more
+// than 9 terms use atanf.
 //
 // The reference's hard cut-off 0 < xi < nCols is a discontinuity: where a pixel sits
 // within 0.01 column of a fan edge the decision is retaken in FP64 (tangent test on
@@ -33,6 +40,7 @@ struct BpArgs {
     float xi_scale, xi_offset;
     int edge_exact;
     double tan_min, tan_max;
+    float atan_c[kBpAtanMax];  // correction polynomial R(s), s = z^2, lowest order first
 };
 
 constexpr int kBpTx = 16, kBpTy = 16;
@@ -53,14 +61,31 @@ __device__ __forceinline__ PixelGeo pixel_geo(const BpArgs &A, int i, int j)
     return p;
 }
 
+// atan(z) for |z| within the fitted range: z + z*s*R(s), s = z^2 (kNc terms of R);
+// kNc = 0 calls atanf.
+template <int kNc>
+__device__ __forceinline__ float fan_atan(const BpArgs &A, float z)
+{
+    if constexpr (kNc == 0) {
+        return atanf(z);
+    } else {
+        const float s = z * z;
+        float r = A.atan_c[kNc - 1];
+#pragma unroll
+        for (int k = kNc - 2; k >= 0; --k) r = fmaf(r, s, A.atan_c[k]);
+        return fmaf(z * s, r, z);
+    }
+}
+
 // Column coordinate xi of the pixel in view v (xi_raw of BP/transform.cu:85), its
 // in-range flag and the FP32 U, t.
+template <int kNc>
 __device__ __forceinline__ bool column_of(const BpArgs &A, const PixelGeo &p, int v, float &xi, float &U, float &t)
 {
     const float2 cs = A.cs[v];
     U = fmaf(-p.y, cs.y, fmaf(-p.x, cs.x, A.R));  // projection onto the iso-ray
     t = fmaf(p.x, cs.y, -p.y * cs.x);             // distance to the centre ray
-    xi = fmaf(atanf(t * __builtin_amdgcn_rcpf(U)), A.xi_scale, A.xi_offset);
+    xi = fmaf(fan_atan<kNc>(A, t * __builtin_amdgcn_rcpf(U)), A.xi_scale, A.xi_offset);
     bool in = xi > 0.0f && xi < (float)A.nCols;
     if (A.edge_exact && (fabsf(xi) < kEdgeBand || fabsf(xi - (float)A.nCols) < kEdgeBand)) {
         const double2 c = A.csd[v];
@@ -81,36 +106,46 @@ __device__ __forceinline__ int interp_pos(float xi, int nCols, float &al)
     return (int)fc;
 }
 
-template <bool kWeighted>
-__global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A)
+// Slices [slice0 + S*blockIdx.z, +S) of the batch.
+template <bool kWeighted, int S, int kNc>
+__global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
 {
     const int i = blockIdx.x * kBpTx + threadIdx.x % kBpTx;
     const int j = blockIdx.y * kBpTy + threadIdx.x / kBpTx;
-    const int sl = blockIdx.z;
+    const size_t sl = (size_t)slice0 + (size_t)S * blockIdx.z;
     if (i >= A.nX || j >= A.nY) return;
     const PixelGeo pg = pixel_geo(A, i, j);
-    float acc = 0.0f;
+    float acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
     if (pg.x * pg.x + pg.y * pg.y <= A.r2) {
-        const float *sino = A.sino + (size_t)sl * A.sino_stride;
+        const float *sino = A.sino + sl * A.sino_stride;
+        const int c1max = A.nCols - 1;
         for (int v = 0; v < A.nViews; ++v) {
             float xi, U, t;
-            if (!column_of(A, pg, v, xi, U, t)) continue;
+            if (!column_of<kNc>(A, pg, v, xi, U, t)) continue;
             float al;
             const int c0 = interp_pos(xi, A.nCols, al);
+            const int c1 = min(c0 + 1, c1max);
             const float *P = sino + (size_t)v * A.nCols;
-            const float p0 = P[c0];
-            const float p1 = P[min(c0 + 1, A.nCols - 1)];
-            const float val = fmaf(al, p1 - p0, p0);
-            acc += kWeighted ? val * __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : val;
+            const float w = kWeighted ? __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : 1.0f;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const float p0 = P[(size_t)s * A.sino_stride + c0];
+                const float p1 = P[(size_t)s * A.sino_stride + c1];
+                const float val = fmaf(al, p1 - p0, p0);
+                acc[s] = kWeighted ? fmaf(val, w, acc[s]) : acc[s] + val;
+            }
         }
     }
-    A.img[(size_t)sl * A.img_stride + (size_t)j * A.nX + i] = acc;
+#pragma unroll
+    for (int s = 0; s < S; ++s) A.img[(sl + s) * A.img_stride + (size_t)j * A.nX + i] = acc[s];
 }
 
 // Transpose: a workgroup owns one detector row (view v, slice) in LDS, visits every
 // pixel of the FOV and scatters the two linear-interpolation weights (clamped taps
 // folded into the edge columns) with ds_add_f32; the row is stored once.
-template <bool kWeighted>
+template <bool kWeighted, int kNc>
 __global__ __launch_bounds__(256) void bp_adjoint_kernel(BpArgs A)
 {
     extern __shared__ float row[];
@@ -126,7 +161,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_kernel(BpArgs A)
         const float val0 = img[p];
         if (val0 == 0.0f) continue;
         float xi, U, t;
-        if (!column_of(A, pg, v, xi, U, t)) continue;
+        if (!column_of<kNc>(A, pg, v, xi, U, t)) continue;
         float al;
         const int c0 = interp_pos(xi, A.nCols, al);
         const float val = kWeighted ? val0 * __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : val0;
@@ -163,7 +198,30 @@ BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, f
     A.edge_exact = p.d.bp_edge_exact;
     A.tan_min = p.d.bp_tan_min;
     A.tan_max = p.d.bp_tan_max;
+    for (int k = 0; k < kBpAtanMax; ++k) A.atan_c[k] = p.d.bp_atan_c[k];
     return A;
+}
+
+template <bool kWeighted, int kNc>
+void launch_bp_groups(const BpArgs &A, dim3 tiles, int n, hipStream_t stream)
+{
+    // slice groups of 8/4/2 while they leave enough workgroups to fill the chip, the
+    // remainder one slice per group
+    const long long nt = (long long)tiles.x * tiles.y;
+    int done = 0;
+    for (int S : {8, 4, 2, 1}) {
+        const int groups = (n - done) / S;
+        if (groups == 0 || (S > 1 && nt * groups < 2048)) continue;
+        const dim3 grid(tiles.x, tiles.y, groups);
+        switch (S) {
+        case 8: hipLaunchKernelGGL((bp_kernel<kWeighted, 8, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        case 4: hipLaunchKernelGGL((bp_kernel<kWeighted, 4, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        case 2: hipLaunchKernelGGL((bp_kernel<kWeighted, 2, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        default: hipLaunchKernelGGL((bp_kernel<kWeighted, 1, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        }
+        done += groups * S;
+        if (done == n) break;
+    }
 }
 
 }  // namespace
@@ -172,11 +230,17 @@ fbct_status launch_bp(const fbct_plan &p, const DeviceTables &t, const float *si
                       bool weighted, hipStream_t stream)
 {
     BpArgs A = make_args(p, t, sino, img);
-    dim3 grid((p.g.nX + kBpTx - 1) / kBpTx, (p.g.nY + kBpTy - 1) / kBpTy, n);
-    if (weighted)
-        hipLaunchKernelGGL(bp_kernel<true>, grid, dim3(kBpTx * kBpTy), 0, stream, A);
-    else
-        hipLaunchKernelGGL(bp_kernel<false>, grid, dim3(kBpTx * kBpTy), 0, stream, A);
+    const dim3 tiles((p.g.nX + kBpTx - 1) / kBpTx, (p.g.nY + kBpTy - 1) / kBpTy);
+    const int nc = p.d.bp_atan_nc;
+    if (weighted) {
+        if (nc == 9) launch_bp_groups<true, 9>(A, tiles, n, stream);
+        else if (nc == 5) launch_bp_groups<true, 5>(A, tiles, n, stream);
+        else launch_bp_groups<true, 0>(A, tiles, n, stream);
+    } else {
+        if (nc == 9) launch_bp_groups<false, 9>(A, tiles, n, stream);
+        else if (nc == 5) launch_bp_groups<false, 5>(A, tiles, n, stream);
+        else launch_bp_groups<false, 0>(A, tiles, n, stream);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
@@ -190,10 +254,18 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     const size_t lds = (size_t)p.g.nCols * sizeof(float);
     if (lds > 64 * 1024) return fail(FBCT_ERR_UNSUPPORTED, "fbct_bp_adjoint: detector row too wide for LDS");
     dim3 grid(p.g.nViews, 1, n);
-    if (weighted)
-        hipLaunchKernelGGL(bp_adjoint_kernel<true>, grid, dim3(256), lds, stream, A);
-    else
-        hipLaunchKernelGGL(bp_adjoint_kernel<false>, grid, dim3(256), lds, stream, A);
+    const int nc = p.d.bp_atan_nc;
+#define FBCT_BPA(W, NC) hipLaunchKernelGGL((bp_adjoint_kernel<W, NC>), grid, dim3(256), lds, stream, A)
+    if (weighted) {
+        if (nc == 9) FBCT_BPA(true, 9);
+        else if (nc == 5) FBCT_BPA(true, 5);
+        else FBCT_BPA(true, 0);
+    } else {
+        if (nc == 9) FBCT_BPA(false, 9);
+        else if (nc == 5) FBCT_BPA(false, 5);
+        else FBCT_BPA(false, 0);
+    }
+#undef FBCT_BPA
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_adjoint_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

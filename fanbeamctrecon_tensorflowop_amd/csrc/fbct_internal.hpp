@@ -19,6 +19,9 @@ constexpr int kFpSlab = 32;
 constexpr int kFpNumCfg = 2;
 constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
 
+// BP atan(z) = z + z^3 R(z^2): at most this many coefficients of R.
+constexpr int kBpAtanMax = 9;
+
 struct FpBundleCfg {
     int rays;  // rays per bundle
     int nb;    // bundles per view
@@ -51,6 +54,8 @@ struct Derived {
     // g = atan(t/U), the pixel is inside iff tan_min < t/U < tan_max.
     int bp_edge_exact;    // 1 when the fan edges lie within (-pi/2, pi/2)
     double bp_tan_min, bp_tan_max;
+    int bp_atan_nc;              // coefficients of R in use (5 or 9), 0 = atanf
+    float bp_atan_c[kBpAtanMax];
     // FP adjoint (gather)
     float fpa_r;          // rFOV/dx
     float fpa_c_scale;    // 1/dCol

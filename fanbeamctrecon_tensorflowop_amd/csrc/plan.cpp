@@ -156,6 +156,46 @@ static double bf16_value(uint16_t b)
     return static_cast<double>(f);
 }
 
+// Fit atan(z) ~ z + z*s*R(s), s = z^2, on |z| <= zmax: R interpolates
+// (atan(sqrt(s))/sqrt(s) - 1)/s at nc Chebyshev nodes of [0, zmax^2] (solved in FP64 in
+// the monomial basis, well conditioned for zmax < 1), coefficients rounded to FP32.
+// Accepted when the relative error of the rounded polynomial, evaluated in FP64 on a
+// dense grid of (0, zmax], is at most 1e-8.
+static bool fit_atan(double zmax, int nc, float *coef)
+{
+    const double smax = zmax * zmax;
+    double a[kBpAtanMax][kBpAtanMax + 1];
+    for (int i = 0; i < nc; ++i) {
+        const double si = smax * 0.5 * (1.0 + std::cos(3.14159265358979323846 * (i + 0.5) / nc));
+        const double zi = std::sqrt(si);
+        double pw = 1.0;
+        for (int k = 0; k < nc; ++k, pw *= si) a[i][k] = pw;
+        a[i][nc] = (std::atan(zi) / zi - 1.0) / si;
+    }
+    for (int c = 0; c < nc; ++c) {  // Gaussian elimination, partial pivoting
+        int piv = c;
+        for (int r = c + 1; r < nc; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0.0) return false;
+        for (int k = 0; k <= nc; ++k) std::swap(a[c][k], a[piv][k]);
+        for (int r = 0; r < nc; ++r) {
+            if (r == c) continue;
+            const double f = a[r][c] / a[c][c];
+            for (int k = c; k <= nc; ++k) a[r][k] -= f * a[c][k];
+        }
+    }
+    for (int k = 0; k < nc; ++k) coef[k] = static_cast<float>(a[k][nc] / a[k][k]);
+    constexpr int kGrid = 4096;
+    for (int i = 1; i <= kGrid; ++i) {
+        const double z = zmax * i / kGrid, sz = z * z;
+        double r = coef[nc - 1];
+        for (int k = nc - 2; k >= 0; --k) r = r * sz + coef[k];
+        const double at = std::atan(z);
+        if (!(std::fabs(z + z * sz * r - at) <= 1e-8 * at)) return false;
+    }
+    return true;
+}
+
 static void build_tables(fbct_plan &p)
 {
     const fbct_fan_geom &g = p.g;
@@ -246,6 +286,21 @@ static void build_tables(fbct_plan &p)
     d.bp_xi_offset = static_cast<float>(static_cast<double>(g.dShift) -
                                         static_cast<double>(g.dLeft) / static_cast<double>(g.dCol));
     d.r2 = g.rFOV * g.rFOV;
+    // FOV pixels see the source at |t/U| <= rFOV/sqrt(R^2 - rFOV^2); the margin covers
+    // the FP32 pixel centres on the circle.
+    {
+        const double rf = static_cast<double>(g.rFOV), R = static_cast<double>(g.R);
+        const double zmax = 1.001 * rf / std::sqrt(R * R - rf * rf);
+        d.bp_atan_nc = 0;
+        std::fill(d.bp_atan_c, d.bp_atan_c + kBpAtanMax, 0.0f);
+        for (int nc : {5, 9}) {
+            if (fit_atan(zmax, nc, d.bp_atan_c)) {
+                d.bp_atan_nc = nc;
+                break;
+            }
+        }
+        if (d.bp_atan_nc == 0) std::fill(d.bp_atan_c, d.bp_atan_c + kBpAtanMax, 0.0f);
+    }
 
     // Ramp taps, FBP/transform.cu:153-162 (Hsieh eq. 3.51, scaled by R*dAngle*dCol).
     p.ramp.assign(static_cast<size_t>(2 * nC - 1), 0.0);
