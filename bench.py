@@ -40,6 +40,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--slices", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("clinical", "uneven"), default="clinical",
+                    help="clinical: 64 slices of 512^2 x 984 views (the headline); uneven: 256 slices of "
+                         "seeded mixed geometry (sides 256-768, 360-1200 views, 512-1024 bins) in one "
+                         "persistent launch per operator, autotuned")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="dispatch each operator once on the batch (plus a 256 MiB calibration copy) "
                          "for counter collection under rocprofv3, then exit")
@@ -57,6 +61,8 @@ def main(argv=None):
     from fanbeamctrecon_tensorflowop_amd.plan import FanBeamPlan
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.workload == "uneven":
+        return uneven_main(args)
     dev = torch.device("cuda:0")
     S = args.slices
     p, a = workloads.clinical_scan(n_slices=S)
@@ -231,6 +237,89 @@ def main(argv=None):
             "value": round(2.0 * U / tc / 1e9, 4), "unit": "GUPS", "cores": nthr, "kind": "port",
             "sample": "FP64 oracle FP + matched adjoint of 1 of the %d slices (%.1f s)" % (S, tc),
         }
+    print(json.dumps(out), flush=True)
+
+
+def uneven_main(args):
+    """Uneven batch: 256 slices, each with its own seeded geometry, FP + matched adjoint
+    of all slices as one persistent launch per operator (launch shape autotuned)."""
+    import numpy as np
+    import torch
+
+    from fanbeamctrecon_tensorflowop_amd import workloads
+    from fanbeamctrecon_tensorflowop_amd.batch import UnevenBatch, autotune
+    from fanbeamctrecon_tensorflowop_amd.plan import FanBeamPlan
+
+    n = 256
+    dev = torch.device("cuda:0")
+    log("bench uneven: %d plans" % n)
+    scans = workloads.uneven_scans(n, seed=4)
+    plans = [FanBeamPlan(p, a) for p, a in scans]
+    batch = UnevenBatch(plans)
+    log("bench uneven: phantoms")
+    phantoms = workloads.uneven_phantoms(scans, seed=4)
+    x = batch.pack_images(phantoms, device=dev)
+    shifted = [np.roll(ph, max(1, ph.shape[1] // 56), axis=1) for ph in phantoms]
+    target = batch.fp(batch.pack_images(shifted, device=dev))
+    xb = x.to(torch.bfloat16)
+    torch.cuda.synchronize()
+    log("bench uneven: autotune")
+    tune = autotune(batch, x, target, reps=2)
+    log("bench uneven: shape %s" % tune["shape"])
+
+    def step_f32():
+        return batch.fp_adjoint(batch.fp(x) - target)
+
+    def step_bf16():
+        return batch.fp_adjoint((batch.fp(xb) - target).to(torch.bfloat16))
+
+    def timed(fn, reps):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f32 = timed(step_f32, args.steps)
+    ms_bf16 = timed(step_bf16, args.steps)
+    ms_fp = timed(lambda: batch.fp(x), max(3, args.steps // 2))
+    ms_adj = timed(lambda: batch.fp_adjoint(target), max(3, args.steps // 2))
+    log("bench uneven: per-slice plain drop-in")
+    xs = [batch.image(x, s)[None] for s in range(n)]
+    ts = [batch.sino(target, s)[None] for s in range(n)]
+
+    def plain():
+        for s in range(n):
+            plans[s].fp_adjoint(plans[s].fp(xs[s]) - ts[s])
+
+    ms_plain = timed(plain, 2)
+    U = batch.updates
+    out = {
+        "metric": "uneven batch FP + matched adjoint: GUPS and ms/slice",
+        "value": round(2.0 * U / (ms_bf16 * 1e-3) / 1e9, 3),
+        "unit": "GUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_bf16, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "uneven batch of %d slices: side {256,384,512,768}, views 360-1200, bins "
+                               "512-1024 (seed 4), FOV 179.2 mm, fan 54.894 deg" % n,
+                   "storage": "bf16 image and residual; f32 arithmetic", "slices": n,
+                   "voxel_view_updates_per_operator": U, "fp_samples": batch.fp_samples},
+        "shape": tune["shape"],
+        "ms_per_slice_bf16": round(ms_bf16 / n, 4),
+        "ms_per_slice_f32": round(ms_f32 / n, 4),
+        "gups_f32": round(2.0 * U / (ms_f32 * 1e-3) / 1e9, 3),
+        "fp_gups": round(U / (ms_fp * 1e-3) / 1e9, 3),
+        "fp_adjoint_gups": round(U / (ms_adj * 1e-3) / 1e9, 3),
+        "ms_per_slice_plain_f32": round(ms_plain / n, 4),
+        "batched_bf16_vs_plain": round(ms_plain / ms_bf16, 3),
+        "batched_f32_vs_plain": round(ms_plain / ms_f32, 3),
+    }
     print(json.dumps(out), flush=True)
 
 

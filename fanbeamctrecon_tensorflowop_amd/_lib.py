@@ -53,6 +53,15 @@ class FanGeom(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchShape(ctypes.Structure):
+    """Mirror of ``fbct_batch_shape``."""
+    _fields_ = [("fp_rays", ctypes.c_int32), ("fp_wgs_per_cu", ctypes.c_int32),
+                ("adj_tile_x", ctypes.c_int32), ("adj_wgs_per_cu", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every symbol the header declares.
 _P = ctypes.c_void_p
 _F = ctypes.POINTER(ctypes.c_float)
@@ -83,6 +92,16 @@ SIGNATURES = [
     ("fbct_fbp", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fbp_adjoint", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fdk", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
+    ("fbct_batch_create", _I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),
+    ("fbct_batch_destroy", _I, [_P]),
+    ("fbct_batch_size", _I, [_P]),
+    ("fbct_batch_layout", _I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    ("fbct_batch_work", _I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    ("fbct_batch_get_shape", _I, [_P, ctypes.POINTER(BatchShape)]),
+    ("fbct_batch_set_shape", _I, [_P, ctypes.POINTER(BatchShape)]),
+    ("fbct_batch_upload", _I, [_P]),
+    ("fbct_batch_fp", _I, [_P, _P, _I, _P, _I, _P]),
+    ("fbct_batch_fp_adjoint", _I, [_P, _P, _I, _P, _I, _P]),
 ]
 
 

@@ -28,6 +28,33 @@ struct FpBundleCfg {
     int cap;   // LDS rows reserved for the minor extent of a slab
 };
 
+// FP^T pixel tile widths of the uneven-batch kernel (tile height = 256 / width).
+constexpr int kAdjNumTiles = 4;
+constexpr int kAdjTileX[kAdjNumTiles] = {16, 32, 8, 64};
+
+// Per-slice descriptors of an uneven batch (device resident).  Offsets count elements
+// of the storage type from the packed buffers' bases.
+struct FpSliceDesc {
+    const float4 *ray_geo;
+    const int32_t *ray_n;
+    const uint8_t *major[kFpNumCfg];
+    long long img_off, sino_off;
+    int nX, nY, nViews, nCols;
+    int nb[kFpNumCfg], cap[kFpNumCfg];
+    int nslab;
+    float scale;
+};
+
+struct FpaSliceDesc {
+    const float4 *ray_geo;
+    const int32_t *ray_n;
+    const float2 *ray_inv;
+    const float2 *cs;
+    long long img_off, sino_off;
+    int nX, nY, nViews, nCols;
+    float scale, r, dx, R, c_scale, c_off, c_abs, far_x, far_y;
+};
+
 // Device-resident copy of a plan's tables on one HIP device.
 struct DeviceTables {
     int device = -1;
@@ -88,6 +115,32 @@ struct fbct_plan {
 
 namespace fbct {
 
+// A batch's work tables on one device.
+struct BatchDevice {
+    int device = -1;
+    int num_cu = 0;
+    FpSliceDesc *fp_desc = nullptr;              // [n]
+    FpaSliceDesc *fpa_desc = nullptr;            // [n]
+    int *fp_prefix[kFpNumCfg] = {};              // [n+1] work items before slice s
+    int *fpa_prefix[kAdjNumTiles] = {};          // [n+1]
+};
+
+}  // namespace fbct
+
+struct fbct_batch {
+    std::vector<fbct_plan *> plans;             // per slice
+    std::vector<int64_t> img_off, sino_off;      // [n+1] element offsets
+    std::vector<int> fp_prefix[fbct::kFpNumCfg];
+    std::vector<int> fpa_prefix[fbct::kAdjNumTiles];
+    int fp_cap_max[fbct::kFpNumCfg] = {};
+    int64_t fp_samples = 0, updates = 0;
+    fbct_batch_shape shape{};
+    std::mutex mu;
+    std::vector<fbct::BatchDevice> dev;
+};
+
+namespace fbct {
+
 // Thread-local error reporting.
 fbct_status fail(fbct_status s, const char *fmt, ...);
 fbct_status ok();
@@ -106,6 +159,10 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
                               float *sino, int n, bool weighted, hipStream_t stream);
 fbct_status launch_ramp_filter(const fbct_plan &p, const DeviceTables &t, const float *in,
                                float *out, int64_t rows, hipStream_t stream);
+fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const void *img, fbct_dtype ti,
+                            void *sino, fbct_dtype to, hipStream_t stream);
+fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, const void *sino, fbct_dtype ti,
+                                    void *img, fbct_dtype to, hipStream_t stream);
 fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in,
                                     float *out, int64_t rows, hipStream_t stream);
 

@@ -18,7 +18,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
+#include <mutex>
 
 #include "dtype.hpp"
 #include "fbct_internal.hpp"
@@ -96,20 +98,23 @@ __device__ __forceinline__ int wave_max(int v)
 // S slices that share the geometry are projected together: the LDS tile interleaves
 // them ([minor][major][slice]) so one sample position, its weights and its LDS
 // address serve S slices, with S-wide LDS reads.
+// Dynamic LDS of the FP kernels: the slab tile at address 0 (so tap offsets need no
+// base add), followed by the per-wave min/max scratch red[2][2][waves].
+extern __shared__ float fp_lds[];
+
+// Bundle b of view v for slices [s0, s0+S): every thread of the workgroup calls it with
+// the same arguments (it contains barriers).
 template <int S, int G, typename TI, typename TO>
-__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
+__device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0, int *err)
 {
     using TapT = typename Taps<S>::T;
-    // Dynamic LDS: the slab tile at address 0 (so tap offsets need no base add),
-    // followed by the per-wave min/max scratch red[2][2][waves].
-    extern __shared__ float tile[];
+    float *tile = fp_lds;
     int(*red)[2][kFpThreads / 64] =
         reinterpret_cast<int(*)[2][kFpThreads / 64]>(tile + (size_t)kPitch * A.cap * S);
 
     const int tid = threadIdx.x;
     constexpr int kRays = kFpThreads / G;
     const int lane = tid % G;
-    const int b = blockIdx.x, v = blockIdx.y, s0 = blockIdx.z * S;
     const int c = b * kRays + tid / G;
     const bool xmajor = A.major[(size_t)v * A.nb + b] != 0;
 
@@ -246,9 +251,78 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
     }
 }
 
+template <int S, int G, typename TI, typename TO>
+__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
+{
+    fp_bundle<S, G, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S, err);
+}
+
+// Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, view,
+// bundle), items of slice s are [prefix[s], prefix[s+1]).  A fixed grid of workgroups
+// strides over all items (item = blockIdx.x + k * gridDim.x), so neighbouring items --
+// the bundles and views of one slice -- run at the same time on different CUs.
+template <int G, typename TI, typename TO>
+__global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc *__restrict__ desc,
+                                                              const int *__restrict__ prefix, int n, int total,
+                                                              const TI *img, TO *sino, int *err)
+{
+    constexpr int cfg = G == 4 ? 0 : 1;
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const FpSliceDesc &d = desc[lo];
+        FpArgs A;
+        A.img = img + d.img_off;
+        A.sino = sino + d.sino_off;
+        A.ray_geo = d.ray_geo;
+        A.ray_n = d.ray_n;
+        A.major = d.major[cfg];
+        A.img_stride = 0;
+        A.sino_stride = 0;
+        A.nX = d.nX;
+        A.nY = d.nY;
+        A.nViews = d.nViews;
+        A.nCols = d.nCols;
+        A.nb = d.nb[cfg];
+        A.nSlices = 1;
+        A.nslab = d.nslab;
+        A.cap = d.cap[cfg];
+        A.scale = d.scale;
+        const int local = item - prefix[lo];
+        fp_bundle<1, G, TI, TO>(A, local % A.nb, local / A.nb, 0, err);
+        __syncthreads();  // the next item reuses the LDS tile and reduction scratch
+    }
+}
+
 __device__ int g_fp_err = 0;
 
 }  // namespace
+
+// Device address of g_fp_err on the current device, looked up once per device (so no
+// runtime symbol lookup happens while a caller captures a graph).
+static fbct_status fp_err_ptr(int **out)
+{
+    static std::mutex mu;
+    static int *ptrs[64] = {};
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= 64) return fail(FBCT_ERR_UNSUPPORTED, "device index %d", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[dev]) {
+        e = hipGetSymbolAddress(reinterpret_cast<void **>(&ptrs[dev]), HIP_SYMBOL(g_fp_err));
+        if (e != hipSuccess) {
+            ptrs[dev] = nullptr;
+            return fail(FBCT_ERR_DEVICE, "hipGetSymbolAddress: %s", hipGetErrorString(e));
+        }
+    }
+    *out = ptrs[dev];
+    return FBCT_OK;
+}
 
 template <int S, int G, typename TI, typename TO>
 static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, int *err)
@@ -286,8 +360,8 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.scale = p.d.fp_scale;
     A.nSlices = n;
     int *err = nullptr;
-    hipError_t e = hipGetSymbolAddress(reinterpret_cast<void **>(&err), HIP_SYMBOL(g_fp_err));
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetSymbolAddress: %s", hipGetErrorString(e));
+    if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
+    hipError_t e;
 
     // Slices per workgroup S and bundle configuration: share each sample's position and
     // weights across 4 slices with 32-ray bundles (smaller tile), else 2 or 1 slices with
@@ -326,6 +400,47 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     if (st != FBCT_OK) return st;
     e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));
+    return FBCT_OK;
+}
+
+template <int G, typename TI, typename TO>
+static void launch_fp_batch_t(dim3 grid, size_t lds, hipStream_t stream, const fbct_batch &b, const BatchDevice &bd,
+                              const void *img, void *sino, int *err)
+{
+    constexpr int cfg = G == 4 ? 0 : 1;
+    const int n = (int)b.plans.size();
+    hipLaunchKernelGGL((fp_batch_kernel<G, TI, TO>), grid, dim3(kFpThreads), lds, stream, bd.fp_desc,
+                       bd.fp_prefix[cfg], n, b.fp_prefix[cfg][n], static_cast<const TI *>(img),
+                       static_cast<TO *>(sino), err);
+}
+
+fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const void *img, fbct_dtype ti, void *sino,
+                            fbct_dtype to, hipStream_t stream)
+{
+    const int n = (int)b.plans.size();
+    const int cfg = b.shape.fp_rays == 32 ? 1 : 0;
+    const int total = b.fp_prefix[cfg][n];
+    if (total == 0) return FBCT_OK;
+    int *err = nullptr;
+    if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
+    hipError_t e;
+    const size_t lds = (size_t)kPitch * b.fp_cap_max[cfg] * sizeof(float) + 2 * 2 * (kFpThreads / 64) * sizeof(int);
+    if (lds > 64 * 1024)
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_batch_fp: slab tile of %zu bytes exceeds 64 KiB", lds);
+    const long long wgs = std::min<long long>(total, (long long)b.shape.fp_wgs_per_cu * bd.num_cu);
+    const dim3 grid((unsigned)wgs);
+#define FBCT_FPB(G)                                                                                       \
+    do {                                                                                                  \
+        if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_batch_t<G, float, float>(grid, lds, stream, b, bd, img, sino, err); \
+        else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_batch_t<G, bf16_t, float>(grid, lds, stream, b, bd, img, sino, err); \
+        else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_batch_t<G, float, bf16_t>(grid, lds, stream, b, bd, img, sino, err); \
+        else launch_fp_batch_t<G, bf16_t, bf16_t>(grid, lds, stream, b, bd, img, sino, err);          \
+    } while (0)
+    if (cfg == 0) FBCT_FPB(4);
+    else FBCT_FPB(8);
+#undef FBCT_FPB
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_batch_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

@@ -21,13 +21,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dtype.hpp"
 #include "fbct_internal.hpp"
 
 namespace fbct {
 namespace {
 
-constexpr int kAdjTx = 16, kAdjTy = 16;
+constexpr int kAdjThreads = 256;
+constexpr int kAdjTx = 16, kAdjTy = kAdjThreads / kAdjTx;  // tile of the regular launches
 constexpr float kBig = 1e30f;
 
 struct FpaArgs {
@@ -135,19 +138,69 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
                     (float)(acc[s] * (double)A.scale));
 }
 
-template <int S, typename TI, typename TO>
-__global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
+// Pixel tile (tx, ty) of TX x (256/TX) pixels for slices [s0, s0+S).
+template <int S, int TX, typename TI, typename TO>
+__device__ __forceinline__ void adjoint_tile(const FpaArgs &A, int tx, int ty, int s0)
 {
-    const int i = blockIdx.x * kAdjTx + threadIdx.x % kAdjTx;
-    const int j = blockIdx.y * kAdjTy + threadIdx.x / kAdjTx;
-    const int s0 = blockIdx.z * S;
+    constexpr int TY = kAdjThreads / TX;
+    const int i = tx * TX + threadIdx.x % TX;
+    const int j = ty * TY + threadIdx.x / TX;
     if (i >= A.nX || j >= A.nY) return;
-    const bool border_tile = blockIdx.x == 0 || blockIdx.y == 0 || (int)(blockIdx.x + 1) * kAdjTx >= A.nX ||
-                             (int)(blockIdx.y + 1) * kAdjTy >= A.nY;
+    const bool border_tile = tx == 0 || ty == 0 || (tx + 1) * TX >= A.nX || (ty + 1) * TY >= A.nY;
     if (border_tile)
         adjoint_pixel<S, true, TI, TO>(A, i, j, s0);
     else
         adjoint_pixel<S, false, TI, TO>(A, i, j, s0);
+}
+
+template <int S, typename TI, typename TO>
+__global__ __launch_bounds__(kAdjThreads) void fp_adjoint_kernel(FpaArgs A)
+{
+    adjoint_tile<S, kAdjTx, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S);
+}
+
+// Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, pixel tile),
+// items of slice s are [prefix[s], prefix[s+1]).  A fixed grid strides over all items.
+template <int TX, typename TI, typename TO>
+__global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const FpaSliceDesc *__restrict__ desc,
+                                                                       const int *__restrict__ prefix, int n,
+                                                                       int total, const TI *sino, TO *img)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const FpaSliceDesc &d = desc[lo];
+        FpaArgs A;
+        A.sino = sino + d.sino_off;
+        A.img = img + d.img_off;
+        A.ray_geo = d.ray_geo;
+        A.ray_n = d.ray_n;
+        A.ray_inv = d.ray_inv;
+        A.cs = d.cs;
+        A.img_stride = 0;
+        A.sino_stride = 0;
+        A.nX = d.nX;
+        A.nY = d.nY;
+        A.nViews = d.nViews;
+        A.nCols = d.nCols;
+        A.nSlices = 1;
+        A.scale = d.scale;
+        A.r = d.r;
+        A.dx = d.dx;
+        A.R = d.R;
+        A.c_scale = d.c_scale;
+        A.c_off = d.c_off;
+        A.c_abs = d.c_abs;
+        A.far_x = d.far_x;
+        A.far_y = d.far_y;
+        const int local = item - prefix[lo];
+        const int ntx = (A.nX + TX - 1) / TX;
+        adjoint_tile<1, TX, TI, TO>(A, local % ntx, local / ntx, 0);
+    }
 }
 
 }  // namespace
@@ -155,7 +208,7 @@ __global__ __launch_bounds__(kAdjTx *kAdjTy) void fp_adjoint_kernel(FpaArgs A)
 template <int S>
 static fbct_status launch_fpa_t(fbct_dtype ti, fbct_dtype to, dim3 grid, hipStream_t stream, const FpaArgs &A)
 {
-    const dim3 block(kAdjTx * kAdjTy);
+    const dim3 block(kAdjThreads);
     if (ti == FBCT_F32 && to == FBCT_F32)
         hipLaunchKernelGGL((fp_adjoint_kernel<S, float, float>), grid, block, 0, stream, A);
     else if (ti == FBCT_BF16 && to == FBCT_F32)
@@ -211,6 +264,47 @@ fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const v
     if (st != FBCT_OK) return st;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_adjoint_kernel launch: %s", hipGetErrorString(e));
+    return FBCT_OK;
+}
+
+template <int TX, typename TI, typename TO>
+static void launch_fpa_batch_t(dim3 grid, hipStream_t stream, const BatchDevice &bd, const int *prefix, int n,
+                               int total, const void *sino, void *img)
+{
+    hipLaunchKernelGGL((fp_adjoint_batch_kernel<TX, TI, TO>), grid, dim3(kAdjThreads), 0, stream, bd.fpa_desc,
+                       prefix, n, total, static_cast<const TI *>(sino), static_cast<TO *>(img));
+}
+
+template <int TX>
+static void launch_fpa_batch_x(fbct_dtype ti, fbct_dtype to, dim3 grid, hipStream_t stream, const BatchDevice &bd,
+                               const int *prefix, int n, int total, const void *sino, void *img)
+{
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fpa_batch_t<TX, float, float>(grid, stream, bd, prefix, n, total, sino, img);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fpa_batch_t<TX, bf16_t, float>(grid, stream, bd, prefix, n, total, sino, img);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fpa_batch_t<TX, float, bf16_t>(grid, stream, bd, prefix, n, total, sino, img);
+    else launch_fpa_batch_t<TX, bf16_t, bf16_t>(grid, stream, bd, prefix, n, total, sino, img);
+}
+
+fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, const void *sino, fbct_dtype ti,
+                                    void *img, fbct_dtype to, hipStream_t stream)
+{
+    const int n = (int)b.plans.size();
+    int k = 0;
+    while (k < kAdjNumTiles && kAdjTileX[k] != b.shape.adj_tile_x) ++k;
+    if (k == kAdjNumTiles) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_batch_fp_adjoint: tile width %d", b.shape.adj_tile_x);
+    const int total = b.fpa_prefix[k][n];
+    if (total == 0) return FBCT_OK;
+    const long long wgs = std::min<long long>(total, (long long)b.shape.adj_wgs_per_cu * bd.num_cu);
+    const dim3 grid((unsigned)wgs);
+    const int *prefix = bd.fpa_prefix[k];
+    switch (b.shape.adj_tile_x) {
+    case 8: launch_fpa_batch_x<8>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
+    case 32: launch_fpa_batch_x<32>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
+    case 64: launch_fpa_batch_x<64>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
+    default: launch_fpa_batch_x<16>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_adjoint_batch_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

@@ -6,6 +6,8 @@ Geometries (16-float parameter vector + view angles, the reference op attributes
   bins of 1e-3 rad (1 mm arc pitch at the detector), 360 views over 360 deg.
 * ``clinical_scan`` 512x512 image, 0.7 mm pixels (FOV radius 179.2 mm), SID 541 mm,
   SDD 949 mm, 888 bins of 1.0239 mm / 949 mm rad, 984 views over 360 deg.
+* ``uneven_scans`` a seeded list of per-slice geometries of different image, view and
+  detector sizes over the clinical FOV and fan (the uneven-batch workload).
 
 Phantoms are sums of uniform ellipses (values added, then clipped), sampled at the
 pixel centres, image layout [nY][nX].
@@ -19,7 +21,7 @@ import numpy as np
 from .compat import angle, make_parameter
 
 __all__ = ["small_scan", "clinical_scan", "scan", "ellipse_phantom", "clinical_phantom",
-           "tiny_scan"]
+           "tiny_scan", "uneven_scans", "uneven_phantoms"]
 
 
 def scan(*, nx, rfov, sid, sdd, n_cols, dcol_rad, n_views, n_slices=1, dshift=0.0, dleft_rad=None,
@@ -42,6 +44,31 @@ def small_scan(n_slices=1):
 def clinical_scan(n_slices=1):
     return scan(nx=512, rfov=179.2, sid=541.0, sdd=949.0, n_cols=888, dcol_rad=1.0239 / 949.0, n_views=984,
                 n_slices=n_slices)
+
+
+CLINICAL_FAN_RAD = 888 * 1.0239 / 949.0  # 54.894 deg
+
+
+def uneven_scans(n_slices=256, seed=4, sides=(256, 384, 512, 768), views=(360, 1200), bins=(512, 1024),
+                 rfov=179.2, sid=541.0, sdd=949.0):
+    """Per-slice (parameter, angles) of an uneven batch: image side uniform in `sides`,
+    views uniform in [views[0], views[1]], bins uniform in [bins[0], bins[1]]; the FOV
+    (radius rfov, pixel size 2 rfov / side) and the fan (54.894 deg, dCol = fan / bins)
+    are fixed so every slice covers the same field."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_slices):
+        side = int(rng.choice(sides))
+        nv = int(rng.integers(views[0], views[1] + 1))
+        nc = int(rng.integers(bins[0], bins[1] + 1))
+        out.append(scan(nx=side, rfov=rfov, sid=sid, sdd=sdd, n_cols=nc, dcol_rad=CLINICAL_FAN_RAD / nc,
+                        n_views=nv))
+    return out
+
+
+def uneven_phantoms(scans, seed=4, rfov=179.2):
+    """One seeded ellipse phantom per slice of `scans` (seed + slice index), float32."""
+    return [ellipse_phantom(int(p[5]), int(p[6]), rfov, 8, seed + 1000 + k) for k, (p, _) in enumerate(scans)]
 
 
 def tiny_scan(nx=16, n_views=24, n_cols=20, n_slices=1, sid=60.0, rfov=8.0, fan_margin=1.15, dshift=0.0):

@@ -195,6 +195,54 @@ fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, i
 fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices,
                      void *workspace, size_t workspace_bytes, void *stream);
 
+/* ------------------------------------------------------------ uneven batches --
+ * No reference counterpart: the reference projects one slice geometry per op call
+ * (FP/transform.cu:173-269 loops its slices with one process-global geometry).  A
+ * batch holds slices of different geometries (image size, views, detector bins), each
+ * slice with its own plan, and runs FP or FP^T over all of them in one persistent
+ * launch: a fixed grid of workgroups strides over the work items (slice, view, ray
+ * bundle) for FP and (slice, pixel tile) for FP^T.  The plans must outlive the batch.
+ *
+ * Data of slice s sits in one packed device buffer per direction:
+ *   images   at element offset image_offsets[s], [nY_s][nX_s]
+ *   sinograms at element offset sino_offsets[s],  [nViews_s][nCols_s]
+ * (fbct_batch_layout; offsets count elements of the storage type). */
+typedef struct fbct_batch fbct_batch;
+
+/* Launch shape of a batch's persistent kernels. */
+typedef struct fbct_batch_shape {
+    int32_t fp_rays;        /* rays per FP workgroup: 64 or 32                       */
+    int32_t fp_wgs_per_cu;  /* FP workgroups per compute unit in the grid, 1..32      */
+    int32_t adj_tile_x;     /* FP^T pixel tile width: 8, 16, 32 or 64 (height 256/w) */
+    int32_t adj_wgs_per_cu; /* FP^T workgroups per compute unit in the grid, 1..32    */
+} fbct_batch_shape;
+
+fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch **out);
+fbct_status fbct_batch_destroy(fbct_batch *batch);
+int fbct_batch_size(const fbct_batch *batch);
+
+/* image_offsets[n_slices+1], sino_offsets[n_slices+1]: element offsets of each slice in
+ * the packed buffers; the last entry is the total element count.  Either may be NULL. */
+fbct_status fbct_batch_layout(const fbct_batch *batch, int64_t *image_offsets, int64_t *sino_offsets);
+
+/* Total FP samples and voxel-view updates (nX*nY*nViews summed over slices). */
+fbct_status fbct_batch_work(const fbct_batch *batch, int64_t *fp_samples, int64_t *updates);
+
+fbct_status fbct_batch_get_shape(const fbct_batch *batch, fbct_batch_shape *shape);
+fbct_status fbct_batch_set_shape(fbct_batch *batch, const fbct_batch_shape *shape);
+
+/* Upload every slice plan's tables and the batch's work tables to the current device
+ * (done implicitly by the first execute; call it before capturing a HIP graph). */
+fbct_status fbct_batch_upload(fbct_batch *batch);
+
+/* FP of every slice: packed images -> packed sinograms (storage types fbct_dtype). */
+fbct_status fbct_batch_fp(fbct_batch *batch, const void *images, int image_dtype, void *sinos, int sino_dtype,
+                          void *stream);
+
+/* Matched adjoint of fbct_batch_fp: packed sinograms -> packed images. */
+fbct_status fbct_batch_fp_adjoint(fbct_batch *batch, const void *sinos, int sino_dtype, void *images,
+                                  int image_dtype, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

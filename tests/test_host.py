@@ -96,6 +96,37 @@ def test_execute_validation_without_gpu():
     assert lib.fbct_ramp_filter(h, fake, fake, 3, null) == 1  # in == out
 
 
+def test_uneven_batch_host_side():
+    """fbct_batch_* creation, layout, work counts and shape validation (no device calls)."""
+    from fanbeamctrecon_tensorflowop_amd.batch import UnevenBatch
+    lib = _lib.load()
+    scans = workloads.uneven_scans(5, seed=3, sides=(16, 24), views=(10, 20), bins=(12, 30), rfov=8.0, sid=60.0,
+                                   sdd=120.0)
+    plans = [FanBeamPlan(p, a) for p, a in scans]
+    b = UnevenBatch(plans)
+    assert lib.fbct_batch_size(b._h) == 5
+    assert list(np.diff(b.image_offsets)) == [p.nx * p.ny for p in plans]
+    assert list(np.diff(b.sino_offsets)) == [p.n_views * p.n_cols for p in plans]
+    assert b.fp_samples == sum(p.fp_samples for p in plans)
+    assert b.updates == sum(p.nx * p.ny * p.n_views for p in plans)
+    assert b.shape == {"fp_rays": 64, "fp_wgs_per_cu": 8, "adj_tile_x": 16, "adj_wgs_per_cu": 8}
+    b.set_shape(fp_rays=32, adj_tile_x=64, fp_wgs_per_cu=1, adj_wgs_per_cu=32)
+    assert b.shape == {"fp_rays": 32, "fp_wgs_per_cu": 1, "adj_tile_x": 64, "adj_wgs_per_cu": 32}
+    for bad in ({"fp_rays": 48}, {"adj_tile_x": 12}, {"fp_wgs_per_cu": 0}, {"adj_wgs_per_cu": 33}):
+        with pytest.raises(_lib.FbctError) as e:
+            b.set_shape(**bad)
+        assert e.value.status == 1
+    assert b.shape["fp_rays"] == 32  # a rejected shape leaves the batch unchanged
+    null = ctypes.c_void_p(0)
+    assert lib.fbct_batch_fp(b._h, null, 0, null, 0, null) == 1
+    assert lib.fbct_batch_fp_adjoint(b._h, ctypes.c_void_p(0x1000), 7, ctypes.c_void_p(0x1000), 0, null) == 1
+    out = ctypes.c_void_p()
+    assert lib.fbct_batch_create(None, 3, ctypes.byref(out)) == 1
+    arr = (ctypes.c_void_p * 2)(plans[0]._h.value, None)
+    assert lib.fbct_batch_create(arr, 2, ctypes.byref(out)) == 1
+    assert lib.fbct_batch_create(arr, 0, ctypes.byref(out)) == 1
+
+
 def test_ray_table_matches_oracle(oracle_mod):
     for p, a in (workloads.small_scan(), workloads.clinical_scan()):
         plan = FanBeamPlan(p, a)
