@@ -14,7 +14,8 @@ python name            registration                         here
 
 Each takes ``(volume|projection, angle, parameter, vol_shape, proj_shape)`` exactly as
 the reference is called (utils.py:25,35,45; FDK/FDK_test_training.py:111) and returns a
-tensor of the attribute-declared output shape (FP/transform.cc:29-33).  Unlike the
+tensor of the attribute-declared output shape (FP/transform.cc:29-33).  They dispatch to
+the custom ops ``torch.ops.fbct.*`` registered in ``torch_ops``.  Unlike the
 reference, which trusts ``parameter[11]`` for the slice count (FP/transform.cc:75), the
 input's element count is checked against it.
 
@@ -29,7 +30,8 @@ import math
 
 import torch
 
-from .plan import FanBeamPlan, get_plan
+from . import torch_ops  # noqa: F401  (registers torch.ops.fbct.*)
+from .plan import FanBeamPlan
 
 __all__ = [
     "fp_fan_curve", "bp_fan_curve", "fbp_fan_curve", "BackProjection",
@@ -112,45 +114,37 @@ def fdk(proj: torch.Tensor, plan: FanBeamPlan) -> torch.Tensor:
 
 # ------------------------------------------------------------------ drop-ins
 
-def _slices(x: torch.Tensor, plan: FanBeamPlan, per_slice: int, what: str) -> torch.Tensor:
-    n = plan.n_samp
-    if n <= 0:
-        raise ValueError("%s: parameter[11] (slice count) must be positive, got %d" % (what, n))
-    if x.numel() != n * per_slice:
-        raise ValueError("%s: input has %d elements but parameter[11]=%d slices need %d"
-                         % (what, x.numel(), n, n * per_slice))
-    return x
+def _attr_list(a):
+    """An op attribute (tensor, array or sequence of floats) as a list of Python floats."""
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().reshape(-1).tolist()
+    elif hasattr(a, "tolist"):
+        a = a.reshape(-1).tolist()
+    return [float(v) for v in a]
 
 
-def _out(y: torch.Tensor, shape, what: str) -> torch.Tensor:
-    shape = tuple(int(s) for s in shape)
-    if math.prod(shape) != y.numel():
-        raise ValueError("%s: declared output shape %s holds %d elements, result has %d"
-                         % (what, shape, math.prod(shape), y.numel()))
-    return y.reshape(shape)
+def _shape_list(s):
+    return [int(v) for v in s]
+
+
+def _call(op, x, angle, parameter, vol_shape, proj_shape):
+    return op(x, _attr_list(angle), _attr_list(parameter), _shape_list(vol_shape), _shape_list(proj_shape))
 
 
 def fp_fan_curve(volume, angle, parameter, vol_shape, proj_shape):
-    """``FPFanCurve`` (FP/transform.cc:17-37,93-117): volume -> projection."""
-    plan = get_plan(parameter, angle)
-    x = _slices(volume, plan, plan.nx * plan.ny, "fp_fan_curve").reshape(plan.n_samp, plan.ny, plan.nx)
-    return _out(forward_project(x, plan), proj_shape, "fp_fan_curve")
+    """``FPFanCurve`` (FP/transform.cc:17-37,93-117): volume -> projection
+    (``torch.ops.fbct.fp_fan_curve``; backward = matched adjoint)."""
+    return _call(torch.ops.fbct.fp_fan_curve, volume, angle, parameter, vol_shape, proj_shape)
 
 
 def bp_fan_curve(projection, angle, parameter, vol_shape, proj_shape):
     """``BPFanCurve`` (BP/transform.cc:17-36,92-116): projection -> volume (unweighted BP)."""
-    plan = get_plan(parameter, angle)
-    p = _slices(projection, plan, plan.n_views * plan.n_cols, "bp_fan_curve")
-    p = p.reshape(plan.n_samp, plan.n_views, plan.n_cols)
-    return _out(back_project(p, plan), vol_shape, "bp_fan_curve")
+    return _call(torch.ops.fbct.bp_fan_curve, projection, angle, parameter, vol_shape, proj_shape)
 
 
 def fbp_fan_curve(projection, angle, parameter, vol_shape, proj_shape):
     """``FBPFanCurve`` (FBP/transform.cc:17-36,92-116): ramp filter + 1/L^2-weighted BP."""
-    plan = get_plan(parameter, angle)
-    p = _slices(projection, plan, plan.n_views * plan.n_cols, "fbp_fan_curve")
-    p = p.reshape(plan.n_samp, plan.n_views, plan.n_cols)
-    return _out(filtered_back_project(p, plan), vol_shape, "fbp_fan_curve")
+    return _call(torch.ops.fbct.fbp_fan_curve, projection, angle, parameter, vol_shape, proj_shape)
 
 
 def BackProjection(projection, angle, parameter, vol_shape, proj_shape):  # noqa: N802 (reference name)
@@ -161,10 +155,4 @@ def BackProjection(projection, angle, parameter, vol_shape, proj_shape):  # noqa
     is taken from the input (all slices are reconstructed); with one slice the result
     is the reference's.
     """
-    plan = get_plan(parameter, angle)
-    per = plan.n_views * plan.n_cols
-    if projection.numel() % per != 0:
-        raise ValueError("BackProjection: input size %d is not a multiple of nViews*nCols=%d"
-                         % (projection.numel(), per))
-    p = projection.reshape(projection.numel() // per, plan.n_views, plan.n_cols)
-    return _out(fdk(p, plan), vol_shape, "BackProjection")
+    return _call(torch.ops.fbct.back_projection, projection, angle, parameter, vol_shape, proj_shape)

@@ -21,7 +21,8 @@ import numpy as np
 from .compat import angle, make_parameter
 
 __all__ = ["small_scan", "clinical_scan", "scan", "ellipse_phantom", "clinical_phantom",
-           "tiny_scan", "uneven_scans", "uneven_phantoms"]
+           "tiny_scan", "uneven_scans", "uneven_phantoms", "two_basis_batch", "poisson_target",
+           "MU_WATER_PER_G_CC", "MU_IODINE_PER_MG_CC"]
 
 
 def scan(*, nx, rfov, sid, sdd, n_cols, dcol_rad, n_views, n_slices=1, dshift=0.0, dleft_rad=None,
@@ -69,6 +70,30 @@ def uneven_scans(n_slices=256, seed=4, sides=(256, 384, 512, 768), views=(360, 1
 def uneven_phantoms(scans, seed=4, rfov=179.2):
     """One seeded ellipse phantom per slice of `scans` (seed + slice index), float32."""
     return [ellipse_phantom(int(p[5]), int(p[6]), rfov, 8, seed + 1000 + k) for k, (p, _) in enumerate(scans)]
+
+
+# Linear attenuation per unit concentration at a 70 keV effective energy (mm^-1), from
+# the NIST XCOM mass attenuation coefficients (water 0.193 cm^2/g; iodine about
+# 4.9 cm^2/g at 70 keV, above its 33.2 keV K-edge).
+MU_WATER_PER_G_CC = 0.0193
+MU_IODINE_PER_MG_CC = 4.9e-4
+
+
+def two_basis_batch(n_slices=16, nx=512, rfov=179.2, seed=100):
+    """Two-basis images: water density U[0, 1.1] g/cc and iodine U[0, 20] mg/cc, each a
+    seeded ellipse phantom (water seeds seed..seed+n-1, iodine seeds seed+n..seed+2n-1).
+    Returns (water [n,nx,nx], iodine [n,nx,nx], mu = c_w water + c_I iodine), float32."""
+    water = np.stack([ellipse_phantom(nx, nx, rfov, 8, seed + s, vmax=1.1) for s in range(n_slices)])
+    iodine = np.stack([ellipse_phantom(nx, nx, rfov, 8, seed + n_slices + s, vmax=20.0) for s in range(n_slices)])
+    mu = (MU_WATER_PER_G_CC * water.astype(np.float64) + MU_IODINE_PER_MG_CC * iodine).astype(np.float32)
+    return water, iodine, mu
+
+
+def poisson_target(line_integrals, photons=1e5, seed=7):
+    """Noisy line integrals -ln(max(Poisson(I0 exp(-p)), 1) / I0), float64."""
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(photons * np.exp(-np.asarray(line_integrals, dtype=np.float64)))
+    return -np.log(np.maximum(counts, 1) / photons)
 
 
 def tiny_scan(nx=16, n_views=24, n_cols=20, n_slices=1, sid=60.0, rfov=8.0, fan_margin=1.15, dshift=0.0):
