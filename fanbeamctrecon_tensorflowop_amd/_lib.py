@@ -62,6 +62,22 @@ class BatchShape(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class ConeGeom(ctypes.Structure):
+    """Mirror of ``fbct_cone_geom``."""
+    _fields_ = [
+        ("R", ctypes.c_float), ("D", ctypes.c_float), ("nViews", ctypes.c_int32), ("dAngle", ctypes.c_float),
+        ("detector", ctypes.c_int32), ("nCols", ctypes.c_int32), ("nRows", ctypes.c_int32),
+        ("dCol", ctypes.c_float), ("dRow", ctypes.c_float), ("colLeft", ctypes.c_float),
+        ("rowBottom", ctypes.c_float), ("dShift", ctypes.c_float), ("nX", ctypes.c_int32),
+        ("nY", ctypes.c_int32), ("nZ", ctypes.c_int32), ("dx", ctypes.c_float), ("dy", ctypes.c_float),
+        ("dz", ctypes.c_float), ("xmin", ctypes.c_float), ("ymin", ctypes.c_float), ("zmin", ctypes.c_float),
+        ("rFOV", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every symbol the header declares.
 _P = ctypes.c_void_p
 _F = ctypes.POINTER(ctypes.c_float)
@@ -102,6 +118,13 @@ SIGNATURES = [
     ("fbct_batch_upload", _I, [_P]),
     ("fbct_batch_fp", _I, [_P, _P, _I, _P, _I, _P]),
     ("fbct_batch_fp_adjoint", _I, [_P, _P, _I, _P, _I, _P]),
+    ("fbct_cone_geom_from_params", _I, [_F, _I, _I, _I, ctypes.POINTER(ConeGeom)]),
+    ("fbct_cone_plan_create", _I, [ctypes.POINTER(ConeGeom), _F, _I, ctypes.POINTER(_P)]),
+    ("fbct_cone_plan_destroy", _I, [_P]),
+    ("fbct_cone_workspace_size", ctypes.c_size_t, [_P, _I]),
+    ("fbct_cone_filter", _I, [_P, _P, _I, _P, _I, _P]),
+    ("fbct_cone_backproject", _I, [_P, _P, _P, _I, _I, _P]),
+    ("fbct_cone_fdk", _I, [_P, _P, _I, _P, _I, _I, _P, ctypes.c_size_t, _P]),
 ]
 
 

@@ -21,19 +21,6 @@ namespace {
 
 constexpr int kMaxWgsPerCu = 32;
 
-template <typename T>
-fbct_status upload_vec(T **dst, const std::vector<T> &src)
-{
-    const size_t bytes = std::max<size_t>(src.size() * sizeof(T), sizeof(T));
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), bytes);
-    if (e != hipSuccess) return fail(FBCT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    if (!src.empty()) {
-        e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
-    }
-    return FBCT_OK;
-}
-
 void free_batch_device(BatchDevice &d)
 {
     int cur = -1;

@@ -127,6 +127,43 @@ struct BatchDevice {
 
 }  // namespace fbct
 
+namespace fbct {
+
+// Scalars of a cone plan's backprojection, derived once in FP64 on the host.
+struct ConeDerived {
+    float xi_scale, xi_offset;  // flat: xi = (t/U) xi_scale + xi_offset; curved: atan(t/U) xi_scale + xi_offset
+    float zr_scale, zr_offset;  // zeta = z (1/U) zr_scale + zr_offset
+    float r2;                   // rFOV^2 (float)
+    int edge_exact;             // 1: the in-range decision near the edges is retaken in FP64 on t/U
+    double tan_min, tan_max;    //    0 < xi < nCols  <=>  tan_min < t/U < tan_max
+    int atan_nc;                // curved: coefficients of the atan correction polynomial (0 = atanf)
+    float atan_c[kBpAtanMax];
+    int ramp_np, ramp_kp;
+};
+
+struct ConeDeviceTables {
+    int device = -1;
+    float2 *view_cs = nullptr;     // [nViews]
+    double2 *view_csd = nullptr;   // [nViews]
+    float *preweight = nullptr;    // [nRows][nCols] (flat detector)
+    uint16_t *ramp_planes = nullptr;  // [3][Np][Kp]
+};
+
+}  // namespace fbct
+
+struct fbct_cone_plan {
+    fbct_cone_geom g{};
+    std::vector<float> angles;
+    std::vector<float> view_cs;
+    std::vector<double> view_csd;
+    std::vector<float> preweight;  // empty for the curved detector
+    std::vector<double> taps;      // 2*nCols-1, scaled
+    std::vector<uint16_t> ramp_planes;
+    fbct::ConeDerived d{};
+    std::mutex mu;
+    std::vector<fbct::ConeDeviceTables> dev;
+};
+
 struct fbct_batch {
     std::vector<fbct_plan *> plans;             // per slice
     std::vector<int64_t> img_off, sino_off;      // [n+1] element offsets
@@ -144,6 +181,29 @@ namespace fbct {
 // Thread-local error reporting.
 fbct_status fail(fbct_status s, const char *fmt, ...);
 fbct_status ok();
+
+// Host helpers shared by the plan kinds (plan.cpp).
+uint16_t bf16_rne(float f);
+double bf16_value(uint16_t b);
+// atan(z) ~ z + z^3 R(z^2) on |z| <= zmax with nc coefficients of R; false if the fit
+// misses 1e-8 relative error.
+bool fit_atan(double zmax, int nc, float *coef);
+// BF16 hi/lo/ll planes [3][np][kp] of the symmetric Toeplitz matrix H[c][m] = taps[c-m+nC-1].
+void toeplitz_planes(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &planes);
+
+// hipMalloc + copy of a host vector (at least one element is allocated).
+template <typename T>
+fbct_status upload_vec(T **dst, const std::vector<T> &src)
+{
+    const size_t bytes = (src.empty() ? 1 : src.size()) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), bytes);
+    if (e != hipSuccess) return fail(FBCT_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    if (!src.empty()) {
+        e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    return FBCT_OK;
+}
 
 // Get (uploading if needed) the tables for the current device.
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
@@ -163,6 +223,17 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
                             void *sino, fbct_dtype to, hipStream_t stream);
 fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, const void *sino, fbct_dtype ti,
                                     void *img, fbct_dtype to, hipStream_t stream);
+// Toeplitz ramp operand of the MFMA filter: BF16 planes [3][Np][Kp] of H (nC x nC).
+struct RampOperand {
+    const uint16_t *planes;
+    int nC, Np, Kp;
+};
+// rows x nC inputs (FP32 or BF16 storage), optionally times w[r % wrows][c], filtered
+// into FP32 outputs.
+fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
+                             const float *w, int wrows, hipStream_t stream);
+fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, void *vol,
+                           fbct_dtype to, int n, hipStream_t stream);
 fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in,
                                     float *out, int64_t rows, hipStream_t stream);
 

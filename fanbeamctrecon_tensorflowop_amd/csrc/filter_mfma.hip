@@ -9,7 +9,8 @@
 //     hi*hi + hi*lo + lo*hi + lo*lo + hi*ll + ll*hi,
 // which carries the same information as an FP32 product (the dropped terms are below
 // 2^-24 relative).  H's three planes are precomputed per plan (padded with zeros to
-// [Np][Kp]); P is split in registers as it is loaded.
+// [Np][Kp]); P is split in registers as it is loaded (optionally multiplied by a
+// per-sample weight first, the cone-beam pre-weight; a BF16 input needs no split).
 //
 // Tiling: a wave computes 16 rows x 64 columns (4 MFMA tiles) over K in steps of 32;
 // a workgroup of 4 waves covers 64 rows.  Operand fragments are loaded straight from
@@ -32,11 +33,12 @@ constexpr int kMfmaK = 32;
 constexpr int kWaveCols = 64;  // 4 tiles of 16
 
 struct MfmaArgs {
-    const float *in;
-    float *out;
+    const void *in;    // TI[rows][nC]
+    float *out;        // [rows][nC]
     const __bf16 *hp;  // [3][Np][Kp] split planes of H
+    const float *w;    // optional per-sample weights [wrows][nC], row r uses w[r % wrows]
     long long rows;
-    int nC, Np, Kp;
+    int nC, Np, Kp, wrows;
 };
 
 __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &ll)
@@ -48,18 +50,25 @@ __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &
     ll = (__bf16)r2;
 }
 
-// kAligned: every row starts 16-byte aligned (nC % 4 == 0, aligned base), so the 8
-// k-values of an A fragment load as two float4.
-template <bool kAligned>
+__device__ __forceinline__ float widen(float x) { return x; }
+__device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// kAligned: every row starts 16-byte aligned, so the 8 k-values of an A fragment load
+// as 16 bytes.  TI: input storage (float or BF16 bits).  kW: inputs are multiplied by a
+// per-sample weight before the split.  A BF16 input without weights is exactly its own
+// hi part, so only the three products with the H planes are formed.
+template <bool kAligned, typename TI, bool kW>
 __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
 {
+    constexpr bool kExactA = sizeof(TI) == 2 && !kW;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const long long r0 = (long long)blockIdx.x * 64 + wave * 16;
     const int c0 = blockIdx.y * kWaveCols;
     const long long arow = r0 + (lane & 15);
     const bool arow_ok = arow < A.rows;
-    const float *prow = A.in + (arow_ok ? arow : 0) * (long long)A.nC;
+    const TI *prow = static_cast<const TI *>(A.in) + (arow_ok ? arow : 0) * (long long)A.nC;
+    const float *wrow = kW ? A.w + (arow_ok ? arow % A.wrows : 0) * (long long)A.nC : nullptr;
     const int kl = 8 * (lane >> 4);  // this lane's k offset within a 32-wide step
     const size_t plane = (size_t)A.Np * A.Kp;
 
@@ -71,20 +80,39 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
         // A fragment: 8 consecutive k of one row, split into three BF16 planes
         float x[8];
         const int kb = k0 + kl;
-        if (kAligned && arow_ok && kb + 8 <= A.nC) {
-            const float4 u = *reinterpret_cast<const float4 *>(prow + kb);
-            const float4 w = *reinterpret_cast<const float4 *>(prow + kb + 4);
-            x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = w.x, x[5] = w.y, x[6] = w.z, x[7] = w.w;
+        const bool full = arow_ok && kb + 8 <= A.nC;
+        if (kAligned && full) {
+            if constexpr (sizeof(TI) == 4) {
+                const float4 u = *reinterpret_cast<const float4 *>(prow + kb);
+                const float4 v = *reinterpret_cast<const float4 *>(prow + kb + 4);
+                x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
+            } else {
+                const uint4 u = *reinterpret_cast<const uint4 *>(prow + kb);
+                const uint32_t q[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    x[2 * j] = __uint_as_float(q[j] << 16);
+                    x[2 * j + 1] = __uint_as_float(q[j] & 0xffff0000u);
+                }
+            }
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = (arow_ok && kb + j < A.nC) ? prow[kb + j] : 0.0f;
+            for (int j = 0; j < 8; ++j) x[j] = (arow_ok && kb + j < A.nC) ? widen(prow[kb + j]) : 0.0f;
+        }
+        if constexpr (kW) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] *= (arow_ok && kb + j < A.nC) ? wrow[kb + j] : 0.0f;
         }
         bf16x8 ah, al, all;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            __bf16 h, l, m;
-            split3(x[j], h, l, m);
-            ah[j] = h, al[j] = l, all[j] = m;
+            if constexpr (kExactA) {
+                ah[j] = (__bf16)x[j];
+            } else {
+                __bf16 h, l, m;
+                split3(x[j], h, l, m);
+                ah[j] = h, al[j] = l, all[j] = m;
+            }
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -93,10 +121,12 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
             const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(A.hp + off);
             const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(A.hp + plane + off);
             const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(A.hp + 2 * plane + off);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(all, bh, acc[t], 0, 0, 0);
+            if constexpr (!kExactA) {
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(all, bh, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+            }
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bll, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
         }
@@ -113,30 +143,54 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
     }
 }
 
+template <typename TI, bool kW>
+void launch_mfma_t(bool aligned, dim3 grid, hipStream_t stream, const MfmaArgs &A)
+{
+    if (aligned)
+        hipLaunchKernelGGL((ramp_filter_mfma_kernel<true, TI, kW>), grid, dim3(256), 0, stream, A);
+    else
+        hipLaunchKernelGGL((ramp_filter_mfma_kernel<false, TI, kW>), grid, dim3(256), 0, stream, A);
+}
+
 }  // namespace
+
+fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
+                             const float *w, int wrows, hipStream_t stream)
+{
+    if (rows <= 0) return FBCT_OK;
+    MfmaArgs A;
+    A.in = in;
+    A.out = out;
+    A.hp = reinterpret_cast<const __bf16 *>(H.planes);
+    A.w = w;
+    A.rows = rows;
+    A.nC = H.nC;
+    A.Np = H.Np;
+    A.Kp = H.Kp;
+    A.wrows = w ? wrows : 1;
+    if (w && wrows <= 0) return fail(FBCT_ERR_INVALID_ARGUMENT, "ramp filter: weight rows must be positive");
+    const long long gx = (rows + 63) / 64;
+    if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
+    dim3 grid((unsigned)gx, (H.nC + kWaveCols - 1) / kWaveCols);
+    const size_t esz = ti == FBCT_BF16 ? 2 : 4;
+    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 && ((size_t)H.nC * esz) % 16 == 0;
+    if (ti == FBCT_BF16) {
+        if (w) launch_mfma_t<uint16_t, true>(aligned, grid, stream, A);
+        else launch_mfma_t<uint16_t, false>(aligned, grid, stream, A);
+    } else {
+        if (w) launch_mfma_t<float, true>(aligned, grid, stream, A);
+        else launch_mfma_t<float, false>(aligned, grid, stream, A);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_filter_mfma_kernel launch: %s", hipGetErrorString(e));
+    return FBCT_OK;
+}
 
 fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in, float *out,
                                     int64_t rows, hipStream_t stream)
 {
-    MfmaArgs A;
-    A.in = in;
-    A.out = out;
-    A.hp = reinterpret_cast<const __bf16 *>(t.ramp_planes);
-    A.rows = rows;
-    A.nC = p.g.nCols;
-    A.Np = p.d.ramp_np;
-    A.Kp = p.d.ramp_kp;
-    const long long gx = (rows + 63) / 64;
-    if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
-    dim3 grid((unsigned)gx, (p.g.nCols + kWaveCols - 1) / kWaveCols);
-    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 && p.g.nCols % 4 == 0;
-    if (aligned)
-        hipLaunchKernelGGL(ramp_filter_mfma_kernel<true>, grid, dim3(256), 0, stream, A);
-    else
-        hipLaunchKernelGGL(ramp_filter_mfma_kernel<false>, grid, dim3(256), 0, stream, A);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_filter_mfma_kernel launch: %s", hipGetErrorString(e));
-    return FBCT_OK;
+    RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp};
+    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, nullptr, 1, stream);
 }
 
 }  // namespace fbct

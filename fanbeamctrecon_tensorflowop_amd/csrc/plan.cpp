@@ -140,7 +140,7 @@ static RayGeo ray_geometry(const fbct_fan_geom &g, double beta, int c)
 }
 
 // Round a float to the nearest-even bfloat16 (raw bits) and back.
-static uint16_t bf16_rne(float f)
+uint16_t bf16_rne(float f)
 {
     uint32_t u;
     std::memcpy(&u, &f, sizeof(u));
@@ -148,7 +148,7 @@ static uint16_t bf16_rne(float f)
     return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-static double bf16_value(uint16_t b)
+double bf16_value(uint16_t b)
 {
     const uint32_t u = static_cast<uint32_t>(b) << 16;
     float f;
@@ -156,12 +156,31 @@ static double bf16_value(uint16_t b)
     return static_cast<double>(f);
 }
 
+void toeplitz_planes(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &planes)
+{
+    planes.assign(static_cast<size_t>(3) * np * kp, 0);
+    const size_t plane = static_cast<size_t>(np) * kp;
+    for (int c = 0; c < nC; ++c)
+        for (int m = 0; m < nC; ++m) {
+            const double x = taps[c - m + nC - 1];
+            const uint16_t hi = bf16_rne(static_cast<float>(x));
+            const double r1 = x - bf16_value(hi);
+            const uint16_t lo = bf16_rne(static_cast<float>(r1));
+            const double r2 = r1 - bf16_value(lo);
+            const uint16_t ll = bf16_rne(static_cast<float>(r2));
+            const size_t o = static_cast<size_t>(c) * kp + m;
+            planes[o] = hi;
+            planes[plane + o] = lo;
+            planes[2 * plane + o] = ll;
+        }
+}
+
 // Fit atan(z) ~ z + z*s*R(s), s = z^2, on |z| <= zmax: R interpolates
 // (atan(sqrt(s))/sqrt(s) - 1)/s at nc Chebyshev nodes of [0, zmax^2] (solved in FP64 in
 // the monomial basis, well conditioned for zmax < 1), coefficients rounded to FP32.
 // Accepted when the relative error of the rounded polynomial, evaluated in FP64 on a
 // dense grid of (0, zmax], is at most 1e-8.
-static bool fit_atan(double zmax, int nc, float *coef)
+bool fit_atan(double zmax, int nc, float *coef)
 {
     const double smax = zmax * zmax;
     double a[kBpAtanMax][kBpAtanMax + 1];
@@ -324,23 +343,7 @@ static void build_tables(fbct_plan &p)
     // zeros to Np = 64-multiple rows and Kp = 32-multiple columns.
     d.ramp_np = (nC + 63) / 64 * 64;
     d.ramp_kp = (nC + 31) / 32 * 32;
-    p.ramp_planes.assign(static_cast<size_t>(3) * d.ramp_np * d.ramp_kp, 0);
-    {
-        const size_t plane = static_cast<size_t>(d.ramp_np) * d.ramp_kp;
-        for (int c = 0; c < nC; ++c)
-            for (int m = 0; m < nC; ++m) {
-                const double x = p.ramp[c - m + nC - 1];
-                const uint16_t hi = bf16_rne(static_cast<float>(x));
-                const double r1 = x - bf16_value(hi);
-                const uint16_t lo = bf16_rne(static_cast<float>(r1));
-                const double r2 = r1 - bf16_value(lo);
-                const uint16_t ll = bf16_rne(static_cast<float>(r2));
-                const size_t o = static_cast<size_t>(c) * d.ramp_kp + m;
-                p.ramp_planes[o] = hi;
-                p.ramp_planes[plane + o] = lo;
-                p.ramp_planes[2 * plane + o] = ll;
-            }
-    }
+    toeplitz_planes(p.ramp, nC, d.ramp_np, d.ramp_kp, p.ramp_planes);
 
     // BP in-range test 0 < xi < nCols <=> g in (dLeft - dShift dCol, dLeft + (nCols - dShift) dCol)
     // (either order when dCol < 0); near the edges it is decided on tangents in FP64.

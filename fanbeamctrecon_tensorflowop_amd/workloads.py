@@ -22,7 +22,8 @@ from .compat import angle, make_parameter
 
 __all__ = ["small_scan", "clinical_scan", "scan", "ellipse_phantom", "clinical_phantom",
            "tiny_scan", "uneven_scans", "uneven_phantoms", "two_basis_batch", "poisson_target",
-           "MU_WATER_PER_G_CC", "MU_IODINE_PER_MG_CC"]
+           "MU_WATER_PER_G_CC", "MU_IODINE_PER_MG_CC", "cone_scan", "random_ellipsoids",
+           "ellipsoid_volume", "ellipsoid_projections"]
 
 
 def scan(*, nx, rfov, sid, sdd, n_cols, dcol_rad, n_views, n_slices=1, dshift=0.0, dleft_rad=None,
@@ -94,6 +95,107 @@ def poisson_target(line_integrals, photons=1e5, seed=7):
     rng = np.random.default_rng(seed)
     counts = rng.poisson(photons * np.exp(-np.asarray(line_integrals, dtype=np.float64)))
     return -np.log(np.maximum(counts, 1) / photons)
+
+
+def cone_scan(*, nx=512, nz=256, dx=0.7, dz=None, n_views=720, n_cols=1024, n_rows=256, pitch=1.0,
+              row_pitch=None, R=541.0, D=949.0, detector="flat", n_views_range=None):
+    """(geometry dict of fbct_cone_geom, angles) for a circular orbit: full 360 degree
+    scan of n_views views, a centred detector of n_cols x n_rows cells (pitch mm for the
+    flat detector at distance D; the curved one spans the same fan), and a centred
+    nx x nx x nz volume of dx x dx x dz voxels (FOV radius nx dx / 2).  The default is
+    the 512 x 512 x 256 volume, 720 views, 1024 x 256 flat panel at 1.0 mm workload."""
+    from .cone import cone_geom
+    dz = dx if dz is None else dz
+    row_pitch = pitch if row_pitch is None else row_pitch
+    if detector == "flat":
+        d_col = pitch
+    else:
+        d_col = 2.0 * math.atan(0.5 * n_cols * pitch / D) / n_cols
+    g = cone_geom(R=R, D=D, n_views=n_views, d_angle=2.0 * math.pi / n_views, detector=detector, n_cols=n_cols,
+                  n_rows=n_rows, d_col=d_col, d_row=row_pitch, nx=nx, nz=nz, dx=dx, dz=dz)
+    return g, angle(n_views, 0.0)
+
+
+def random_ellipsoids(n=10, seed=5, rfov=179.2, zhalf=89.6, vmax=0.04):
+    """Seeded ellipsoids [n, 8] = (cx, cy, cz, a, b, c, phi, value): centres uniform in a
+    cylinder of radius 0.6 rfov and half-height 0.6 zhalf, semi-axes U[0.05, 0.35] rfov
+    (c: U[0.1, 0.6] zhalf), rotation phi about z uniform in [0, pi), value U[0, vmax]."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((n, 8))
+    for k in range(n):
+        rr = 0.6 * rfov * math.sqrt(rng.uniform())
+        th = rng.uniform(0, 2 * math.pi)
+        out[k, 0:3] = rr * math.cos(th), rr * math.sin(th), rng.uniform(-0.6, 0.6) * zhalf
+        out[k, 3:5] = rng.uniform(0.05, 0.35, size=2) * rfov
+        out[k, 5] = rng.uniform(0.1, 0.6) * zhalf
+        out[k, 6] = rng.uniform(0, math.pi)
+        out[k, 7] = rng.uniform(0, vmax)
+    return out
+
+
+def ellipsoid_volume(ells, geom):
+    """Sum of the ellipsoids' values at the voxel centres, float32 [nZ, nY, nX]."""
+    g = geom
+    x = g["xmin"] + (np.arange(g["nX"]) + 0.5) * g["dx"]
+    y = g["ymin"] + (np.arange(g["nY"]) + 0.5) * g["dy"]
+    X, Y = np.meshgrid(x, y)
+    vol = np.zeros((g["nZ"], g["nY"], g["nX"]), dtype=np.float32)
+    for k in range(g["nZ"]):
+        z = g["zmin"] + (k + 0.5) * g["dz"]
+        sl = np.zeros((g["nY"], g["nX"]))
+        for cx, cy, cz, a, b, c, phi, val in ells:
+            if abs(z - cz) >= c:
+                continue
+            cp, sp = math.cos(phi), math.sin(phi)
+            xr = (X - cx) * cp + (Y - cy) * sp
+            yr = -(X - cx) * sp + (Y - cy) * cp
+            sl[(xr / a) ** 2 + (yr / b) ** 2 + ((z - cz) / c) ** 2 <= 1.0] += val
+        vol[k] = sl
+    return vol
+
+
+def ellipsoid_projections(ells, geom, angles, device="cpu"):
+    """Exact line integrals of the ellipsoids for every detector cell centre, float64
+    [nViews, nRows, nCols] (a torch tensor on `device`).
+
+    Source at R (cos b, sin b, 0); the cell (u, v) lies at S + D e_c + u e_u + v e_z with
+    e_c = -(cos b, sin b, 0), e_u = (sin b, -cos b, 0): the flat panel of the cone
+    geometry; for the curved detector u = D tan(gamma) of the column's fan angle."""
+    import torch
+    g = geom
+    dt = torch.float64
+    nC, nR = g["nCols"], g["nRows"]
+    cpos = g["colLeft"] + (torch.arange(nC, dtype=dt, device=device) + 0.5 - g["dShift"]) * g["dCol"]
+    u = cpos if g["detector"] == 0 else g["D"] * torch.tan(cpos)
+    v = g["rowBottom"] + (torch.arange(nR, dtype=dt, device=device) + 0.5) * g["dRow"]
+    V, Uc = torch.meshgrid(v, u, indexing="ij")
+    out = torch.zeros((len(angles), nR, nC), dtype=dt, device=device)
+    R, D = float(g["R"]), float(g["D"])
+    for vi, b in enumerate(np.asarray(angles, dtype=np.float64)):
+        cb, sb = math.cos(float(np.float32(b))), math.sin(float(np.float32(b)))
+        S = torch.tensor([R * cb, R * sb, 0.0], dtype=dt, device=device)
+        # direction to the cell: D e_c + u e_u + v e_z
+        dx_ = -D * cb + Uc * sb
+        dy_ = -D * sb - Uc * cb
+        dz_ = V
+        nrm = torch.sqrt(dx_ * dx_ + dy_ * dy_ + dz_ * dz_)
+        dx_, dy_, dz_ = dx_ / nrm, dy_ / nrm, dz_ / nrm
+        acc = torch.zeros_like(nrm)
+        for cx, cy, cz, a, bb, c, phi, val in ells:
+            cp, sp = math.cos(phi), math.sin(phi)
+            # ellipsoid frame: rotate by -phi about z, scale by the semi-axes
+            ox, oy, oz = float(S[0]) - cx, float(S[1]) - cy, -cz
+            o1, o2, o3 = (ox * cp + oy * sp) / a, (-ox * sp + oy * cp) / bb, oz / c
+            d1 = (dx_ * cp + dy_ * sp) / a
+            d2 = (-dx_ * sp + dy_ * cp) / bb
+            d3 = dz_ / c
+            A = d1 * d1 + d2 * d2 + d3 * d3
+            B = 2.0 * (d1 * o1 + d2 * o2 + d3 * o3)
+            C = o1 * o1 + o2 * o2 + o3 * o3 - 1.0
+            disc = B * B - 4.0 * A * C
+            acc += val * torch.where(disc > 0, torch.sqrt(torch.clamp(disc, min=0.0)) / A, torch.zeros_like(A))
+        out[vi] = acc
+    return out
 
 
 def tiny_scan(nx=16, n_views=24, n_cols=20, n_slices=1, sid=60.0, rfov=8.0, fan_margin=1.15, dshift=0.0):

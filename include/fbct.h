@@ -243,6 +243,76 @@ fbct_status fbct_batch_fp(fbct_batch *batch, const void *images, int image_dtype
 fbct_status fbct_batch_fp_adjoint(fbct_batch *batch, const void *sinos, int sino_dtype, void *images,
                                   int image_dtype, void *stream);
 
+/* ---------------------------------------------------------------- cone beam --
+ * Circular-orbit cone-beam FDK with nRows detector rows and an nZ-slice volume, on a
+ * flat or a curved (equiangular) detector.
+ *
+ * The reference's cone kernel (BackProjOneView_CB_EA_kernel, FDK/GPURecon_GPU.cu:15-50,
+ * driven by BP_One_Rot_3D_GPU :163-268) is equiangular and already loops over z with
+ * the row index ((z - zs) D / U - detBottom) / dRow, but its op pins nZ = nRows = 1
+ * (FDK/FBP.cc:85,98).  FBCT_DETECTOR_CURVED is that kernel with any nZ and nRows
+ * (reference filter, no pre-weight, weight 1/(U^2+t^2));  FBCT_DETECTOR_FLAT is the
+ * textbook Feldkamp-Davis-Kress algorithm for a flat panel (no reference counterpart):
+ * cosine pre-weight D / sqrt(D^2+u^2+v^2), Ram-Lak row filter h[0] = 1/(4 dCol^2),
+ * h[odd n] = -1/(pi^2 n^2 dCol^2) scaled by dCol dAngle R D / 2, weight 1/U^2.
+ *
+ * For voxel (x, y, z) and view angle b (source at R (cos b, sin b, 0)):
+ *   U = R - x cos b - y sin b,  t = x sin b - y cos b,
+ *   column xi = (D t/U - colLeft)/dCol + dShift (flat) or (atan(t/U) - colLeft)/dCol + dShift
+ *   (curved), row zeta = (D z/U - rowBottom)/dRow, texel centres at c + 1/2, r + 1/2,
+ *   bilinear with clamp-to-edge; a view counts only where 0 < xi < nCols; voxels with
+ *   x^2 + y^2 > rFOV^2 are 0.
+ * Layouts: projections [n][nViews][nRows][nCols], volumes [n][nZ][nY][nX]. */
+typedef enum fbct_detector {
+    FBCT_DETECTOR_FLAT = 0,
+    FBCT_DETECTOR_CURVED = 1
+} fbct_detector;
+
+typedef struct fbct_cone_geom {
+    float R, D;           /* source-isocentre, source-detector distance (mm)        */
+    int32_t nViews;
+    float dAngle;         /* view spacing in the filter scale (rad)                  */
+    int32_t detector;     /* fbct_detector                                           */
+    int32_t nCols, nRows;
+    float dCol;           /* column pitch: mm (flat) or rad (curved)                 */
+    float dRow;           /* row pitch at the detector (mm)                          */
+    float colLeft;        /* left edge of column 0: mm (flat) or rad (curved)        */
+    float rowBottom;      /* lower edge of row 0 at the detector (mm)                */
+    float dShift;         /* column shift (columns)                                  */
+    int32_t nX, nY, nZ;
+    float dx, dy, dz;     /* voxel size (mm)                                         */
+    float xmin, ymin, zmin; /* volume corner (mm)                                    */
+    float rFOV;           /* reconstructed cylinder radius (mm)                      */
+} fbct_cone_geom;
+
+typedef struct fbct_cone_plan fbct_cone_plan;
+
+/* The reference FDK op's geometry (FDK/FBP.cc:53-123) with nRows rows and nZ slices:
+ * curved detector, colLeft = dLeft, rowBottom = detBottom, dRow, dz from parameter[13..15],
+ * zmin = -nZ dz / 2, dAngle = parameter[3]/nViews in rad. */
+fbct_status fbct_cone_geom_from_params(const float *parameter, int n_parameter, int n_rows, int n_z,
+                                       fbct_cone_geom *out);
+
+fbct_status fbct_cone_plan_create(const fbct_cone_geom *geom, const float *angles, int n_angles,
+                                  fbct_cone_plan **out);
+fbct_status fbct_cone_plan_destroy(fbct_cone_plan *plan);
+
+/* Workspace bytes for fbct_cone_fdk on n volumes (the filtered projections, FP32). */
+size_t fbct_cone_workspace_size(const fbct_cone_plan *plan, int n);
+
+/* Row filter alone (pre-weight + ramp): proj (FP32 or BF16 storage) -> FP32. */
+fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dtype, float *filtered, int n,
+                             void *stream);
+
+/* Backprojection of filtered projections: FP32 -> volume (FP32 or BF16 storage). */
+fbct_status fbct_cone_backproject(fbct_cone_plan *plan, const float *filtered, void *vol, int vol_dtype, int n,
+                                  void *stream);
+
+/* FDK: filter + backprojection.  proj [n][nViews][nRows][nCols] (fbct_dtype) ->
+ * vol [n][nZ][nY][nX] (fbct_dtype); workspace of fbct_cone_workspace_size bytes. */
+fbct_status fbct_cone_fdk(fbct_cone_plan *plan, const void *proj, int proj_dtype, void *vol, int vol_dtype, int n,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

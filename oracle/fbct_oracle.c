@@ -513,3 +513,139 @@ void orc_fdk(const orc_geom *g, const float *angles, const double *proj, double 
     orc_parallel_for((long long)nS * g->nY, nthreads, fdk_item, &x);
     free(q);
 }
+
+/* ------------------------------------------------------------ cone beam */
+
+void orc_cone_taps(const orc_cone *g, double *h)
+{
+    const int nC = g->nCols;
+    const double dc = g->dCol;
+    if (g->detector == 0) {
+        /* discrete Ram-Lak of the flat detector, times dCol (the convolution integral),
+         * dAngle (the view integral), R D (magnification) and 1/2 (every ray twice) */
+        const double pi = 3.14159265358979323846;
+        const double kappa = 0.5 * dc * g->dAngle * g->R * g->D;
+        for (int n = -(nC - 1); n <= nC - 1; ++n) {
+            double v;
+            if (n == 0) v = 1.0 / (4.0 * dc * dc);
+            else if (n % 2 != 0) v = -1.0 / (pi * pi * (double)n * (double)n * dc * dc);
+            else v = 0.0;
+            h[n + nC - 1] = v * kappa;
+        }
+    } else {
+        /* the reference taps, FDK/GPURecon_GPU.cu:190-200 */
+        const double kappa = g->R * g->dAngle * dc;
+        for (int n = -(nC - 1); n <= nC - 1; ++n) {
+            double v;
+            if (n == 0) v = 0.5 / (4.0 * dc * dc);
+            else if (n % 2 != 0) {
+                const double s = sin((double)n * dc);
+                v = -0.5 / (REF_PI_D * REF_PI_D * s * s);
+            } else v = 0.0;
+            h[n + nC - 1] = v * kappa;
+        }
+    }
+}
+
+void orc_cone_preweight(const orc_cone *g, double *w)
+{
+    for (int r = 0; r < g->nRows; ++r)
+        for (int c = 0; c < g->nCols; ++c) {
+            double v = 1.0;
+            if (g->detector == 0) {
+                const double u = g->colLeft + ((double)c + 0.5 - g->dShift) * g->dCol;
+                const double vv = g->rowBottom + ((double)r + 0.5) * g->dRow;
+                v = g->D / sqrt(g->D * g->D + u * u + vv * vv);
+            }
+            w[(long long)r * g->nCols + c] = v;
+        }
+}
+
+typedef struct {
+    const orc_cone *g;
+    const float *angles;
+    const double *in;
+    double *out;
+    const double *h, *w;
+} orc_cone_ctx;
+
+static void cone_filt_item(void *vctx, long long line)
+{
+    const orc_cone_ctx *x = (const orc_cone_ctx *)vctx;
+    const int nC = x->g->nCols, r = (int)(line % x->g->nRows);
+    const double *p = x->in + line * (long long)nC;
+    const double *w = x->w + (long long)r * nC;
+    double *q = x->out + line * (long long)nC;
+    for (int c = 0; c < nC; ++c) {
+        double acc = 0.0;
+        for (int m = 0; m < nC; ++m) acc += p[m] * w[m] * x->h[c - m + nC - 1];
+        q[c] = acc;
+    }
+}
+
+void orc_cone_filter(const orc_cone *g, const double *proj, double *q, int nS, int nthreads)
+{
+    double *h = (double *)malloc(sizeof(double) * (2 * g->nCols - 1));
+    double *w = (double *)malloc(sizeof(double) * (size_t)g->nRows * g->nCols);
+    orc_cone_taps(g, h);
+    orc_cone_preweight(g, w);
+    orc_cone_ctx x = {g, NULL, proj, q, h, w};
+    orc_parallel_for((long long)nS * g->nViews * g->nRows, nthreads, cone_filt_item, &x);
+    free(h);
+    free(w);
+}
+
+static void cone_bp_item(void *vctx, long long item)
+{
+    const orc_cone_ctx *x = (const orc_cone_ctx *)vctx;
+    const orc_cone *g = x->g;
+    const int nC = g->nCols, nR = g->nRows;
+    const int j = (int)(item % g->nY);
+    const long long s = item / g->nY;
+    const double *q = x->in + s * (long long)g->nViews * nR * nC;
+    double *vol = x->out + s * (long long)g->nZ * g->nY * g->nX;
+    const double y = g->ymin + g->dy * ((double)j + 0.5);
+    for (int i = 0; i < g->nX; ++i)
+        for (int k = 0; k < g->nZ; ++k) vol[((long long)k * g->nY + j) * g->nX + i] = 0.0;
+    for (int i = 0; i < g->nX; ++i) {
+        const double xx = g->xmin + g->dx * ((double)i + 0.5);
+        if (!(xx * xx + y * y <= g->rFOV * g->rFOV)) continue;
+        for (int v = 0; v < g->nViews; ++v) {
+            const double beta = (double)x->angles[v];
+            const double cb = cos(beta), sb = sin(beta);
+            const double U = g->R - xx * cb - y * sb;
+            const double t = xx * sb - y * cb;
+            const double xi = (g->detector == 0 ? (g->D * t / U - g->colLeft) : (atan(t / U) - g->colLeft)) / g->dCol +
+                              g->dShift;
+            if (!(xi > 0.0 && xi < (double)nC)) continue;
+            const double wgt = g->detector == 0 ? 1.0 / (U * U) : 1.0 / (U * U + t * t);
+            const double ct = xi - 0.5, fc = floor(ct), al = ct - fc;
+            const int ca = clampi((int)fc, 0, nC - 1), cb2 = clampi((int)fc + 1, 0, nC - 1);
+            const double *V = q + (long long)v * nR * nC;
+            for (int k = 0; k < g->nZ; ++k) {
+                const double z = g->zmin + g->dz * ((double)k + 0.5);
+                const double rt = (g->D * z / U - g->rowBottom) / g->dRow - 0.5;
+                const double fr = floor(rt), be = rt - fr;
+                const int ra = clampi((int)fr, 0, nR - 1), rb = clampi((int)fr + 1, 0, nR - 1);
+                const double top = (1.0 - al) * V[(long long)ra * nC + ca] + al * V[(long long)ra * nC + cb2];
+                const double bot = (1.0 - al) * V[(long long)rb * nC + ca] + al * V[(long long)rb * nC + cb2];
+                vol[((long long)k * g->nY + j) * g->nX + i] += ((1.0 - be) * top + be * bot) * wgt;
+            }
+        }
+    }
+}
+
+void orc_cone_bp(const orc_cone *g, const float *angles, const double *q, double *vol, int nS, int nthreads)
+{
+    orc_cone_ctx x = {g, angles, q, vol, NULL, NULL};
+    orc_parallel_for((long long)nS * g->nY, nthreads, cone_bp_item, &x);
+}
+
+void orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS, int nthreads)
+{
+    const long long n = (long long)nS * g->nViews * g->nRows * g->nCols;
+    double *q = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    orc_cone_filter(g, proj, q, nS, nthreads);
+    orc_cone_bp(g, angles, q, vol, nS, nthreads);
+    free(q);
+}

@@ -105,6 +105,60 @@ void orc_fbp_adjoint(const orc_geom *g, const float *angles, const double *img,
 void orc_fdk(const orc_geom *g, const float *angles, const double *proj, double *vol,
              int nS, int nRows, int nZ, double zmin, int nthreads);
 
+/* ------------------------------------------------------------ cone beam --
+ * Circular-orbit cone-beam FDK on a flat or curved (equiangular) detector with nRows
+ * rows and an nZ-slice volume.  No reference counterpart for the flat detector (the
+ * reference kernel FDK/GPURecon_GPU.cu:15-50 is equiangular and its op pins
+ * nZ = nRows = 1); the curved mode is that kernel generalised, as orc_fdk.
+ *
+ * Geometry (all double; callers widen the float values the GPU plan receives):
+ *   source at R (cos b, sin b, 0); for voxel (x, y, z):
+ *     U = R - x cos b - y sin b,  t = x sin b - y cos b;
+ *   column coordinate  flat:   xi = (D t / U - colLeft) / dCol + dShift   (dCol, colLeft in mm)
+ *                      curved: xi = (atan(t / U) - colLeft) / dCol + dShift (rad)
+ *   row coordinate     zeta = (D z / U - rowBottom) / dRow                 (mm)
+ *   texel centres at xi = c + 1/2, zeta = r + 1/2; bilinear, clamp-to-edge;
+ *   a view contributes only where 0 < xi < nCols (as the reference);
+ *   voxels with x^2 + y^2 > rFOV^2 are 0.
+ * Flat (Feldkamp-Davis-Kress, textbook): each detector sample is pre-weighted by
+ *   D / sqrt(D^2 + u^2 + v^2) at its centre (u, v), rows are convolved with the
+ *   discrete Ram-Lak kernel h[0] = 1/(4 dCol^2), h[odd n] = -1/(pi^2 n^2 dCol^2),
+ *   scaled by dCol * dAngle * R * D / 2, and backprojected with weight 1/U^2.
+ * Curved: the reference filter (orc_ramp_taps, no pre-weight) and weight 1/(U^2 + t^2). */
+typedef struct orc_cone {
+    double R, D;          /* source-isocentre, source-detector distance (mm)    */
+    int nViews;
+    double dAngle;        /* view spacing of the filter scale (rad)              */
+    int detector;         /* 0 flat, 1 curved (equiangular)                      */
+    int nCols, nRows;
+    double dCol;          /* flat: mm; curved: rad                               */
+    double dRow;          /* mm                                                  */
+    double colLeft;       /* left edge of column 0: flat mm, curved rad          */
+    double rowBottom;     /* lower edge of row 0 (mm)                            */
+    double dShift;        /* columns                                             */
+    int nX, nY, nZ;
+    double dx, dy, dz;
+    double xmin, ymin, zmin;
+    double rFOV;
+} orc_cone;
+
+/* Filter taps h[n + nCols - 1], n in [-(nCols-1), nCols-1], with the constant scale. */
+void orc_cone_taps(const orc_cone *g, double *h);
+
+/* Pre-weight w[r][c] (1 for the curved detector). */
+void orc_cone_preweight(const orc_cone *g, double *w);
+
+/* proj[nS][nViews][nRows][nCols] -> filtered q (same layout). */
+void orc_cone_filter(const orc_cone *g, const double *proj, double *q, int nS, int nthreads);
+
+/* Backprojection of filtered q -> vol[nS][nZ][nY][nX]. */
+void orc_cone_bp(const orc_cone *g, const float *angles, const double *q, double *vol, int nS,
+                 int nthreads);
+
+/* FDK = orc_cone_filter + orc_cone_bp. */
+void orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS,
+                  int nthreads);
+
 /* Threads the oracle will use when nthreads <= 0. */
 int orc_default_threads(void);
 

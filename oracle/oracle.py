@@ -33,6 +33,17 @@ class OrcGeom(ctypes.Structure):
     ]
 
 
+class OrcCone(ctypes.Structure):
+    _fields_ = [
+        ("R", ctypes.c_double), ("D", ctypes.c_double), ("nViews", ctypes.c_int), ("dAngle", ctypes.c_double),
+        ("detector", ctypes.c_int), ("nCols", ctypes.c_int), ("nRows", ctypes.c_int), ("dCol", ctypes.c_double),
+        ("dRow", ctypes.c_double), ("colLeft", ctypes.c_double), ("rowBottom", ctypes.c_double),
+        ("dShift", ctypes.c_double), ("nX", ctypes.c_int), ("nY", ctypes.c_int), ("nZ", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("xmin", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("zmin", ctypes.c_double), ("rFOV", ctypes.c_double),
+    ]
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (gcc only)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -71,6 +82,17 @@ def _load():
     lib.orc_ramp_filter.restype = None
     lib.orc_fdk.argtypes = [g, f, d, d, i, i, i, ctypes.c_double, i]
     lib.orc_fdk.restype = None
+    c = P(OrcCone)
+    lib.orc_cone_taps.argtypes = [c, d]
+    lib.orc_cone_taps.restype = None
+    lib.orc_cone_preweight.argtypes = [c, d]
+    lib.orc_cone_preweight.restype = None
+    lib.orc_cone_filter.argtypes = [c, d, d, i, i]
+    lib.orc_cone_filter.restype = None
+    for name in ("orc_cone_bp", "orc_cone_fdk"):
+        fn = getattr(lib, name)
+        fn.argtypes = [c, f, d, d, i, i]
+        fn.restype = None
     lib.orc_default_threads.argtypes = []
     lib.orc_default_threads.restype = i
     _lib = lib
@@ -198,4 +220,62 @@ class Oracle:
         out = np.zeros((nS, n_z) + self.img_shape, dtype=np.float64)
         _load().orc_fdk(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), nS, n_rows, n_z,
                         float(zmin), nthreads)
+        return out
+
+
+class ConeOracle:
+    """FP64 cone-beam FDK oracle for one geometry (a dict with the fields of
+    ``fbct_cone_geom``; values are taken as float32, as the GPU plan receives them)."""
+
+    _INT = ("nViews", "detector", "nCols", "nRows", "nX", "nY", "nZ")
+
+    def __init__(self, geom: dict, angles):
+        _load()
+        vals = {k: (int(v) if k in self._INT else float(np.float32(v))) for k, v in geom.items()}
+        self.g = OrcCone(**vals)
+        self.angles = np.ascontiguousarray(np.asarray(angles, dtype=np.float32).reshape(-1))
+        if self.angles.size != self.g.nViews:
+            raise ValueError("angle list length %d != nViews %d" % (self.angles.size, self.g.nViews))
+
+    @property
+    def proj_shape(self):
+        return (self.g.nViews, self.g.nRows, self.g.nCols)
+
+    @property
+    def vol_shape(self):
+        return (self.g.nZ, self.g.nY, self.g.nX)
+
+    def _proj(self, p):
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+        if p.ndim == 3:
+            p = p[None]
+        assert p.shape[1:] == self.proj_shape, (p.shape, self.proj_shape)
+        return p
+
+    def taps(self):
+        h = np.zeros(2 * self.g.nCols - 1)
+        _load().orc_cone_taps(ctypes.byref(self.g), _dptr(h))
+        return h
+
+    def preweight(self):
+        w = np.zeros((self.g.nRows, self.g.nCols))
+        _load().orc_cone_preweight(ctypes.byref(self.g), _dptr(w))
+        return w
+
+    def filter(self, proj, nthreads=0):
+        p = self._proj(proj)
+        out = np.zeros_like(p)
+        _load().orc_cone_filter(ctypes.byref(self.g), _dptr(p), _dptr(out), p.shape[0], nthreads)
+        return out
+
+    def backproject(self, q, nthreads=0):
+        q = self._proj(q)
+        out = np.zeros((q.shape[0],) + self.vol_shape)
+        _load().orc_cone_bp(ctypes.byref(self.g), _fptr(self.angles), _dptr(q), _dptr(out), q.shape[0], nthreads)
+        return out
+
+    def fdk(self, proj, nthreads=0):
+        p = self._proj(proj)
+        out = np.zeros((p.shape[0],) + self.vol_shape)
+        _load().orc_cone_fdk(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0], nthreads)
         return out
