@@ -201,11 +201,11 @@ fbct_status check_dtype(int d, const char *what)
 }
 
 fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void *proj, fbct_dtype ti, float *q, int n,
-                       hipStream_t st)
+                       bool pairs, hipStream_t st)
 {
     RampOperand H{t.ramp_planes, p->g.nCols, p->d.ramp_np, p->d.ramp_kp};
     const int64_t rows = (int64_t)n * p->g.nViews * p->g.nRows;
-    return launch_ramp_gemm(H, proj, ti, q, rows, t.preweight, p->g.nRows, st);
+    return launch_ramp_gemm(H, proj, ti, q, rows, t.preweight, p->g.nRows, st, pairs);
 }
 
 }  // namespace
@@ -287,7 +287,7 @@ fbct_status fbct_cone_plan_destroy(fbct_cone_plan *plan)
 size_t fbct_cone_workspace_size(const fbct_cone_plan *plan, int n)
 {
     if (!plan || n <= 0) return 0;
-    return fbct::proj_elems(plan, n) * sizeof(float);
+    return fbct::proj_elems(plan, n) * 2 * sizeof(float);
 }
 
 fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dtype, float *filtered, int n,
@@ -298,7 +298,7 @@ fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dt
     if ((s = fbct::check_dtype(proj_dtype, "fbct_cone_filter")) != FBCT_OK) return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), filtered, n,
+    return fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), filtered, n, false,
                             static_cast<hipStream_t>(stream));
 }
 
@@ -310,7 +310,7 @@ fbct_status fbct_cone_backproject(fbct_cone_plan *plan, const float *filtered, v
     if ((s = fbct::check_dtype(vol_dtype, "fbct_cone_backproject")) != FBCT_OK) return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::launch_cone_bp(*plan, *t, filtered, vol, static_cast<fbct_dtype>(vol_dtype), n,
+    return fbct::launch_cone_bp(*plan, *t, filtered, false, vol, static_cast<fbct_dtype>(vol_dtype), n,
                                 static_cast<hipStream_t>(stream));
 }
 
@@ -330,8 +330,11 @@ fbct_status fbct_cone_fdk(fbct_cone_plan *plan, const void *proj, int proj_dtype
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float *q = static_cast<float *>(workspace);
-    if ((s = fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), q, n, st)) != FBCT_OK) return s;
-    return fbct::launch_cone_bp(*plan, *t, q, vol, static_cast<fbct_dtype>(vol_dtype), n, st);
+    // the filter writes column pairs, so the backprojection reads both column taps of a
+    // row with one 8-byte load
+    if ((s = fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), q, n, true, st)) != FBCT_OK)
+        return s;
+    return fbct::launch_cone_bp(*plan, *t, q, true, vol, static_cast<fbct_dtype>(vol_dtype), n, st);
 }
 
 }  // extern "C"

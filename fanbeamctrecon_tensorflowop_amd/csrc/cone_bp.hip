@@ -24,13 +24,13 @@ constexpr int kTx = 16, kTy = 16;
 constexpr float kEdgeBand = 0.01f;  // columns
 
 struct ConeArgs {
-    const float *q;        // [n][nViews][nRows][nCols] filtered projections
+    const float *q;        // [n][nViews][nRows][nCols] filtered projections (float2 pairs: see kPairs)
     void *vol;             // TO[n][nZ][nY][nX]
     const float2 *cs;
     const double2 *csd;
     int nX, nY, nZ, nViews, nCols, nRows, nzc;
     float R, xmin, ymin, zmin, dx, dy, dz, r2;
-    float xi_scale, xi_offset, zr_scale, zr_offset;
+    float xi_scale, xi_offset, zr_scale, zr_offset;  // zr_offset includes the -1/2 texel shift
     double tan_min, tan_max;
     float atan_c[kBpAtanMax];
 };
@@ -51,7 +51,9 @@ __device__ __forceinline__ float fan_atan(const ConeArgs &A, float z)
 
 // kFlat: flat detector (xi linear in t/U, weight 1/U^2); else curved (atan, 1/(U^2+t^2)).
 // kNc: atan polynomial terms (curved only).  KZ: voxels per thread along z.
-template <bool kFlat, int kNc, int KZ, typename TO>
+// kPairs: q holds column pairs (q[c], q[min(c+1, nC-1)]) as float2, so the two column
+// taps of a row are one 8-byte load.
+template <bool kFlat, int kNc, int KZ, bool kPairs, typename TO>
 __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
 {
     const int i = blockIdx.x * kTx + threadIdx.x % kTx;
@@ -75,8 +77,9 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
         const float cmax = (float)(nC - 1), rmax = (float)(nR - 1);
         const float cfl = (float)max(nC - 2, 0), rfl = (float)max(nR - 2, 0);
         const size_t view_elems = (size_t)nR * nC;
-        const float *qv = A.q + vi * (size_t)A.nViews * view_elems;
-        for (int v = 0; v < A.nViews; ++v, qv += view_elems) {
+        constexpr int kStride = kPairs ? 2 : 1;  // floats per detector sample
+        const float *qv = A.q + vi * (size_t)A.nViews * view_elems * kStride;
+        for (int v = 0; v < A.nViews; ++v, qv += view_elems * kStride) {
             const float2 cs = A.cs[v];
             const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
             const float t = fmaf(x, cs.y, -y * cs.x);
@@ -99,18 +102,45 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
             const int c0 = (int)fc;
             const int c1 = min(c0 + 1, nC - 1);
             const float a = rU * A.zr_scale;
+            // row coordinate rt_k = z_k a + zr_offset is monotone in k: when the first and
+            // last voxel of the chunk lie inside [0, nRows-1) no row clamp is needed
+            const float rt_a = fmaf(zk[0], a, A.zr_offset), rt_b = fmaf(zk[KZ - 1], a, A.zr_offset);
+            const bool inside = fminf(rt_a, rt_b) >= 0.0f && fmaxf(rt_a, rt_b) < rmax;
+            auto tap_rows = [&](int r0, int r1, float &top, float &bot) {
+                if constexpr (kPairs) {
+                    const float2 u = *reinterpret_cast<const float2 *>(qv + 2 * (r0 * nC + c0));
+                    const float2 d = *reinterpret_cast<const float2 *>(qv + 2 * (r1 * nC + c0));
+                    top = fmaf(al, u.y - u.x, u.x);
+                    bot = fmaf(al, d.y - d.x, d.x);
+                } else {
+                    const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
+                    const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
+                    top = fmaf(al, p01 - p00, p00);
+                    bot = fmaf(al, p11 - p10, p10);
+                }
+            };
+            if (inside) {
 #pragma unroll
-            for (int k = 0; k < KZ; ++k) {
-                const float rt = fminf(fmaxf(fmaf(zk[k], a, A.zr_offset) - 0.5f, 0.0f), rmax);
-                const float fr = fminf(floorf(rt), rfl);
-                const float be = rt - fr;
-                const int r0 = (int)fr;
-                const int o0 = r0 * nC, o1 = min(r0 + 1, nR - 1) * nC;
-                const float p00 = qv[o0 + c0], p01 = qv[o0 + c1];
-                const float p10 = qv[o1 + c0], p11 = qv[o1 + c1];
-                const float top = fmaf(al, p01 - p00, p00);
-                const float bot = fmaf(al, p11 - p10, p10);
-                acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
+                for (int k = 0; k < KZ; ++k) {
+                    const float rt = fmaf(zk[k], a, A.zr_offset);
+                    const float fr = floorf(rt);
+                    const float be = rt - fr;
+                    const int r0 = (int)fr;
+                    float top, bot;
+                    tap_rows(r0, r0 + 1, top, bot);
+                    acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < KZ; ++k) {
+                    const float rt = fminf(fmaxf(fmaf(zk[k], a, A.zr_offset), 0.0f), rmax);
+                    const float fr = fminf(floorf(rt), rfl);
+                    const float be = rt - fr;
+                    const int r0 = (int)fr;
+                    float top, bot;
+                    tap_rows(r0, min(r0 + 1, nR - 1), top, bot);
+                    acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
+                }
             }
         }
     }
@@ -120,27 +150,36 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
         if (k0 + k < A.nZ) store_f(out, (size_t)(k0 + k) * A.nY * A.nX, acc[k]);
 }
 
-template <bool kFlat, int kNc, int KZ>
+template <bool kFlat, int kNc, int KZ, bool kPairs>
 void launch_kz(dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
     if (to == FBCT_BF16)
-        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, kPairs, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
     else
-        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, float>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, kPairs, float>), grid, dim3(kTx * kTy), 0, st, A);
 }
 
-template <bool kFlat, int kNc>
+template <bool kFlat, int kNc, bool kPairs>
 void launch_nc(int kz, dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
-    if (kz == 16) launch_kz<kFlat, kNc, 16>(grid, st, A, to);
-    else if (kz == 4) launch_kz<kFlat, kNc, 4>(grid, st, A, to);
-    else launch_kz<kFlat, kNc, 1>(grid, st, A, to);
+    if (kz == 16) launch_kz<kFlat, kNc, 16, kPairs>(grid, st, A, to);
+    else if (kz == 4) launch_kz<kFlat, kNc, 4, kPairs>(grid, st, A, to);
+    else launch_kz<kFlat, kNc, 1, kPairs>(grid, st, A, to);
+}
+
+template <bool kPairs>
+void launch_det(const fbct_cone_plan &p, int kz, dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
+{
+    if (p.g.detector == FBCT_DETECTOR_FLAT) launch_nc<true, 0, kPairs>(kz, grid, st, A, to);
+    else if (p.d.atan_nc == 9) launch_nc<false, 9, kPairs>(kz, grid, st, A, to);
+    else if (p.d.atan_nc == 5) launch_nc<false, 5, kPairs>(kz, grid, st, A, to);
+    else launch_nc<false, 0, kPairs>(kz, grid, st, A, to);
 }
 
 }  // namespace
 
-fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, void *vol,
-                           fbct_dtype to, int n, hipStream_t stream)
+fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, bool pairs,
+                           void *vol, fbct_dtype to, int n, hipStream_t stream)
 {
     const fbct_cone_geom &g = p.g;
     ConeArgs A;
@@ -165,7 +204,7 @@ fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, c
     A.xi_scale = p.d.xi_scale;
     A.xi_offset = p.d.xi_offset;
     A.zr_scale = p.d.zr_scale;
-    A.zr_offset = p.d.zr_offset;
+    A.zr_offset = p.d.zr_offset - 0.5f;
     A.tan_min = p.d.tan_min;
     A.tan_max = p.d.tan_max;
     for (int k = 0; k < kBpAtanMax; ++k) A.atan_c[k] = p.d.atan_c[k];
@@ -174,15 +213,8 @@ fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, c
     const long long gz = (long long)A.nzc * n;
     if (gz > 65535) return fail(FBCT_ERR_UNSUPPORTED, "cone backprojection: %lld z-groups exceed the grid", gz);
     const dim3 grid((g.nX + kTx - 1) / kTx, (g.nY + kTy - 1) / kTy, (unsigned)gz);
-    if (g.detector == FBCT_DETECTOR_FLAT) {
-        launch_nc<true, 0>(kz, grid, stream, A, to);
-    } else if (p.d.atan_nc == 9) {
-        launch_nc<false, 9>(kz, grid, stream, A, to);
-    } else if (p.d.atan_nc == 5) {
-        launch_nc<false, 5>(kz, grid, stream, A, to);
-    } else {
-        launch_nc<false, 0>(kz, grid, stream, A, to);
-    }
+    if (pairs) launch_det<true>(p, kz, grid, stream, A, to);
+    else launch_det<false>(p, kz, grid, stream, A, to);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "cone_bp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

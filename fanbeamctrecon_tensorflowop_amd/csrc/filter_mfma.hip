@@ -57,7 +57,9 @@ __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uin
 // as 16 bytes.  TI: input storage (float or BF16 bits).  kW: inputs are multiplied by a
 // per-sample weight before the split.  A BF16 input without weights is exactly its own
 // hi part, so only the three products with the H planes are formed.
-template <bool kAligned, typename TI, bool kW>
+// kPairs: the output is written as column pairs, out2[r][c] = (q[r][c], q[r][min(c+1, nC-1)])
+// (float2), the layout the cone backprojection reads its two column taps from.
+template <bool kAligned, typename TI, bool kW, bool kPairs>
 __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
 {
     constexpr bool kExactA = sizeof(TI) == 2 && !kW;
@@ -138,24 +140,39 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long long row = r0 + 4 * (lane >> 4) + j;
-            if (row < A.rows) A.out[row * A.nC + col] = acc[t][j];
+            if (row >= A.rows) continue;
+            if constexpr (kPairs) {
+                float *o = A.out + 2 * (row * A.nC + col);
+                o[0] = acc[t][j];
+                if (col > 0) o[-1] = acc[t][j];
+                if (col == A.nC - 1) o[1] = acc[t][j];
+            } else {
+                A.out[row * A.nC + col] = acc[t][j];
+            }
         }
     }
 }
 
-template <typename TI, bool kW>
+template <typename TI, bool kW, bool kPairs>
 void launch_mfma_t(bool aligned, dim3 grid, hipStream_t stream, const MfmaArgs &A)
 {
     if (aligned)
-        hipLaunchKernelGGL((ramp_filter_mfma_kernel<true, TI, kW>), grid, dim3(256), 0, stream, A);
+        hipLaunchKernelGGL((ramp_filter_mfma_kernel<true, TI, kW, kPairs>), grid, dim3(256), 0, stream, A);
     else
-        hipLaunchKernelGGL((ramp_filter_mfma_kernel<false, TI, kW>), grid, dim3(256), 0, stream, A);
+        hipLaunchKernelGGL((ramp_filter_mfma_kernel<false, TI, kW, kPairs>), grid, dim3(256), 0, stream, A);
+}
+
+template <typename TI, bool kW>
+void launch_mfma_p(bool pairs, bool aligned, dim3 grid, hipStream_t stream, const MfmaArgs &A)
+{
+    if (pairs) launch_mfma_t<TI, kW, true>(aligned, grid, stream, A);
+    else launch_mfma_t<TI, kW, false>(aligned, grid, stream, A);
 }
 
 }  // namespace
 
 fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
-                             const float *w, int wrows, hipStream_t stream)
+                             const float *w, int wrows, hipStream_t stream, bool pairs)
 {
     if (rows <= 0) return FBCT_OK;
     MfmaArgs A;
@@ -175,11 +192,11 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     const size_t esz = ti == FBCT_BF16 ? 2 : 4;
     const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 && ((size_t)H.nC * esz) % 16 == 0;
     if (ti == FBCT_BF16) {
-        if (w) launch_mfma_t<uint16_t, true>(aligned, grid, stream, A);
-        else launch_mfma_t<uint16_t, false>(aligned, grid, stream, A);
+        if (w) launch_mfma_p<uint16_t, true>(pairs, aligned, grid, stream, A);
+        else launch_mfma_p<uint16_t, false>(pairs, aligned, grid, stream, A);
     } else {
-        if (w) launch_mfma_t<float, true>(aligned, grid, stream, A);
-        else launch_mfma_t<float, false>(aligned, grid, stream, A);
+        if (w) launch_mfma_p<float, true>(pairs, aligned, grid, stream, A);
+        else launch_mfma_p<float, false>(pairs, aligned, grid, stream, A);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_filter_mfma_kernel launch: %s", hipGetErrorString(e));
@@ -190,7 +207,7 @@ fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, c
                                     int64_t rows, hipStream_t stream)
 {
     RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp};
-    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, nullptr, 1, stream);
+    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, nullptr, 1, stream, false);
 }
 
 }  // namespace fbct

@@ -120,3 +120,14 @@ def test_volumes_batch_equals_single(flat_case):
     for k in range(2):
         one = plan.fdk(x[k:k + 1])
         assert rel_l2(both[k].double().cpu().numpy(), one[0].double().cpu().numpy()) <= 1e-6
+
+
+def test_filter_then_backproject_equals_fdk(flat_case):
+    # fbct_cone_fdk reads column pairs written by the filter; the separate entry points
+    # use the plain layout: both give the same volume
+    g, a, P, o, ref = flat_case
+    plan = ConePlan(g, a)
+    x = torch.from_numpy(P).cuda()
+    two_step = plan.backproject(plan.filter(x))
+    one_step = plan.fdk(x)
+    assert rel_l2(two_step.double().cpu().numpy(), one_step.double().cpu().numpy()) <= 1e-6
