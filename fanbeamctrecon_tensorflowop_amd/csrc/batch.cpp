@@ -60,6 +60,7 @@ fbct_status batch_device(fbct_batch *b, const BatchDevice **out)
             f.major[k] = t->fp_major[k];
             f.nb[k] = p->d.fpb[k].nb;
             f.cap[k] = p->d.fpb[k].cap;
+            f.nslab[k] = p->d.fpb[k].nslab;
         }
         f.img_off = b->img_off[s];
         f.sino_off = b->sino_off[s];
@@ -67,7 +68,6 @@ fbct_status batch_device(fbct_batch *b, const BatchDevice **out)
         f.nY = p->g.nY;
         f.nViews = p->g.nViews;
         f.nCols = p->g.nCols;
-        f.nslab = p->d.fp_nslab;
         f.scale = p->d.fp_scale;
         FpaSliceDesc &a = fpad[s];
         a.ray_geo = t->ray_geo;

@@ -12,20 +12,24 @@
 
 namespace fbct {
 
-// FP ray-march tiling (see fp.hip): rays per workgroup bundle and slab width in pixels.
+// FP ray-march tiling (see fp.hip).  Bundle configurations: rays per workgroup (lanes
+// per ray = kFpThreads / rays) and slab width in pixels along the major axis.  The
+// 32-ray, 20-pixel configuration serves the 4-slice tiles: its smaller LDS tile keeps
+// five workgroups resident per CU.
 constexpr int kFpThreads = 256;
-constexpr int kFpSlab = 32;
-// Bundle configurations: rays per workgroup (lanes per ray = kFpThreads / rays).
 constexpr int kFpNumCfg = 2;
 constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
+constexpr int kFpCfgSlab[kFpNumCfg] = {32, 20};
 
 // BP atan(z) = z + z^3 R(z^2): at most this many coefficients of R.
 constexpr int kBpAtanMax = 9;
 
 struct FpBundleCfg {
-    int rays;  // rays per bundle
-    int nb;    // bundles per view
-    int cap;   // LDS rows reserved for the minor extent of a slab
+    int rays;   // rays per bundle
+    int slab;   // slab width (pixels)
+    int nslab;  // slabs along the major axis
+    int nb;     // bundles per view
+    int cap;    // LDS rows reserved for the minor extent of a slab
 };
 
 // FP^T pixel tile widths of the uneven-batch kernel (tile height = 256 / width).
@@ -40,8 +44,7 @@ struct FpSliceDesc {
     const uint8_t *major[kFpNumCfg];
     long long img_off, sino_off;
     int nX, nY, nViews, nCols;
-    int nb[kFpNumCfg], cap[kFpNumCfg];
-    int nslab;
+    int nb[kFpNumCfg], cap[kFpNumCfg], nslab[kFpNumCfg];
     float scale;
 };
 
@@ -72,7 +75,6 @@ struct DeviceTables {
 struct Derived {
     float fp_scale;       // 0.125 * dx
     float two_r;          // 2 * rFOV/dx (FP sample coordinate extent)
-    int fp_nslab;         // slabs along the major axis
     FpBundleCfg fpb[kFpNumCfg];
     float bp_xi_scale;    // 1/dCol
     float bp_xi_offset;   // dShift - dLeft/dCol

@@ -28,7 +28,10 @@
 namespace fbct {
 namespace {
 
-constexpr int kPitch = kFpSlab + 1;  // LDS row: taps [alo, alo + kFpSlab]
+// Bundle configuration of a lane count G (kFpCfgRays[cfg] = kFpThreads / G) and its LDS
+// row pitch: taps [alo, alo + slab] of a slab.
+constexpr int fp_cfg(int G) { return kFpThreads / G == kFpCfgRays[0] ? 0 : 1; }
+constexpr int fp_pitch(int cfg) { return kFpCfgSlab[cfg] + 1; }
 
 struct FpArgs {
     const void *img;   // TI[n][nY][nX]
@@ -108,6 +111,8 @@ template <int S, int G, typename TI, typename TO>
 __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0, int *err)
 {
     using TapT = typename Taps<S>::T;
+    constexpr int kSlab = kFpCfgSlab[fp_cfg(G)];
+    constexpr int kPitch = kSlab + 1;
     float *tile = fp_lds;
     int(*red)[2][kFpThreads / 64] =
         reinterpret_cast<int(*)[2][kFpThreads / 64]>(tile + (size_t)kPitch * A.cap * S);
@@ -151,8 +156,8 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     using LdsTap = __attribute__((address_space(3))) const TapT;
 
     for (int sl = 0; sl < A.nslab; ++sl) {
-        const int alo = sl * kFpSlab - 1;
-        const int knext = slab_boundary(a0, da, N, (float)(alo + kFpSlab));
+        const int alo = sl * kSlab - 1;
+        const int knext = slab_boundary(a0, da, N, (float)(alo + kSlab));
         const int ka = asc ? kprev : knext;
         const int kb = asc ? knext : kprev;
         kprev = knext;
@@ -184,7 +189,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
             continue;
         }
 
-        // Stage taps [alo, alo+kFpSlab] x [mlo, mhi] of the S slices with clamp-to-edge
+        // Stage taps [alo, alo+kSlab] x [mlo, mhi] of the S slices with clamp-to-edge
         // addressing.  Global loads run along the image row (i) in both orientations.
         const int F = xmajor ? kPitch : M;  // extent of the fast (row-contiguous) index
         const int total = F * (xmajor ? M : kPitch);
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
                                                               const int *__restrict__ prefix, int n, int total,
                                                               const TI *img, TO *sino, int *err)
 {
-    constexpr int cfg = G == 4 ? 0 : 1;
+    constexpr int cfg = fp_cfg(G);
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
         while (lo < hi) {
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.nCols = d.nCols;
         A.nb = d.nb[cfg];
         A.nSlices = 1;
-        A.nslab = d.nslab;
+        A.nslab = d.nslab[cfg];
         A.cap = d.cap[cfg];
         A.scale = d.scale;
         const int local = item - prefix[lo];
@@ -356,7 +361,6 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.nY = p.g.nY;
     A.nViews = p.g.nViews;
     A.nCols = p.g.nCols;
-    A.nslab = p.d.fp_nslab;
     A.scale = p.d.fp_scale;
     A.nSlices = n;
     int *err = nullptr;
@@ -368,7 +372,7 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     // 64-ray bundles, while the grid keeps >= 2048 workgroups and the tile <= 80 KB.
     const size_t red_bytes = 2 * 2 * (kFpThreads / 64) * sizeof(int);
     const size_t kMaxTile = 80 * 1024;
-    auto tile_bytes = [&](int cfg, int S) { return (size_t)kPitch * p.d.fpb[cfg].cap * sizeof(float) * S; };
+    auto tile_bytes = [&](int cfg, int S) { return (size_t)fp_pitch(cfg) * p.d.fpb[cfg].cap * sizeof(float) * S; };
     auto grid_wgs = [&](int cfg, int S) { return (long long)p.d.fpb[cfg].nb * p.g.nViews * ((n + S - 1) / S); };
     int S = 1, cfg = 0;
     if (n >= 4 && tile_bytes(1, 4) <= kMaxTile && grid_wgs(1, 4) >= 2048) {
@@ -384,6 +388,7 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.major = t.fp_major[cfg];
     A.nb = bc.nb;
     A.cap = bc.cap;
+    A.nslab = bc.nslab;
     const size_t lds = tile_bytes(cfg, S) + red_bytes;
     if (lds > 160 * 1024)
         return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds);
@@ -407,7 +412,7 @@ template <int G, typename TI, typename TO>
 static void launch_fp_batch_t(dim3 grid, size_t lds, hipStream_t stream, const fbct_batch &b, const BatchDevice &bd,
                               const void *img, void *sino, int *err)
 {
-    constexpr int cfg = G == 4 ? 0 : 1;
+    constexpr int cfg = fp_cfg(G);
     const int n = (int)b.plans.size();
     hipLaunchKernelGGL((fp_batch_kernel<G, TI, TO>), grid, dim3(kFpThreads), lds, stream, bd.fp_desc,
                        bd.fp_prefix[cfg], n, b.fp_prefix[cfg][n], static_cast<const TI *>(img),
@@ -424,7 +429,7 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
     int *err = nullptr;
     if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
     hipError_t e;
-    const size_t lds = (size_t)kPitch * b.fp_cap_max[cfg] * sizeof(float) + 2 * 2 * (kFpThreads / 64) * sizeof(int);
+    const size_t lds = (size_t)fp_pitch(cfg) * b.fp_cap_max[cfg] * sizeof(float) + 2 * 2 * (kFpThreads / 64) * sizeof(int);
     if (lds > 64 * 1024)
         return fail(FBCT_ERR_UNSUPPORTED, "fbct_batch_fp: slab tile of %zu bytes exceeds 64 KiB", lds);
     const long long wgs = std::min<long long>(total, (long long)b.shape.fp_wgs_per_cu * bd.num_cu);

@@ -258,11 +258,13 @@ static void build_tables(fbct_plan &p)
     Derived &d = p.d;
     d.fp_scale = static_cast<float>(0.125 * static_cast<double>(g.dx));
     d.two_r = static_cast<float>(2.0 * r);
-    d.fp_nslab = static_cast<int>(std::floor((2.0 * r + 1.0) / kFpSlab)) + 1;
     for (int cfg = 0; cfg < kFpNumCfg; ++cfg) {
         const int rays = kFpCfgRays[cfg];
         FpBundleCfg &bc = d.fpb[cfg];
+        const int slab = kFpCfgSlab[cfg];
         bc.rays = rays;
+        bc.slab = slab;
+        bc.nslab = static_cast<int>(std::floor((2.0 * r + 1.0) / slab)) + 1;
         bc.nb = (nC + rays - 1) / rays;
         std::vector<uint8_t> &major = p.fp_major[cfg];
         major.assign(static_cast<size_t>(nV) * bc.nb, 1);
@@ -294,7 +296,7 @@ static void build_tables(fbct_plan &p)
                     hi_b = std::max(hi_b, mb);
                 }
                 const double spread = std::max(hi_a - lo_a, hi_b - lo_b);
-                cap = std::max(cap, spread + kFpSlab * slope_max + 4.0);
+                cap = std::max(cap, spread + slab * slope_max + 4.0);
             }
         }
         bc.cap = static_cast<int>(std::ceil(std::min(cap, std::ceil(2.0 * r) + 3.0)));
