@@ -15,9 +15,9 @@
 // centre, rho = sqrt(2) inside the image), found from the pixel's fan angle; per
 // candidate ray the sample range whose positions fall in the support is computed in
 // closed form and the samples are evaluated at exactly the FP32 positions the forward
-// kernel interpolates at.  A group of S slices that share the geometry is processed
-// together: the per-(pixel, view, ray) weight W_c is computed once and applied to S
-// sinograms.  Sums over views accumulate in FP64.
+// kernel interpolates at.  A group of S <= 16 slices that share the geometry is
+// processed together: the per-(pixel, view, ray) weight W_c is computed once and applied
+// to S sinograms.  Sums over views accumulate in FP64.
 
 #include <hip/hip_runtime.h>
 
@@ -253,7 +253,9 @@ fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const v
     // workgroups left to fill the device.
     const long long tiles = (long long)gx * gy;
     fbct_status st;
-    if (n >= 8 && tiles * ((n + 7) / 8) >= 1024)
+    if (n >= 16 && tiles * ((n + 15) / 16) >= 1024)
+        st = launch_fpa_t<16>(ti, to, dim3(gx, gy, (n + 15) / 16), stream, A);
+    else if (n >= 8 && tiles * ((n + 7) / 8) >= 1024)
         st = launch_fpa_t<8>(ti, to, dim3(gx, gy, (n + 7) / 8), stream, A);
     else if (n >= 4 && tiles * ((n + 3) / 4) >= 1024)
         st = launch_fpa_t<4>(ti, to, dim3(gx, gy, (n + 3) / 4), stream, A);
