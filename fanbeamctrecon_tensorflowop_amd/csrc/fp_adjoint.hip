@@ -101,16 +101,20 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
         for (int s = 0; s < S; ++s) vsum[s] = 0.0f;
         const size_t rbase = (size_t)v * A.nCols;
         for (int c = clo; c <= chi; ++c) {
+            // The S sinogram values are loaded first, unconditionally, so their latency
+            // overlaps the weight computation (a ray that misses the support, or has no
+            // samples, ends with wsum = 0; sinograms are expected to be finite).
             const int N = A.ray_n[rbase + c];
-            if (N <= 0) continue;
             const float4 g = A.ray_geo[rbase + c];
             const float2 inv = A.ray_inv[rbase + c];
+            float sv[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) sv[s] = load_f(sino[s], rbase + c);
             const float X0 = g.x - fi, Y0 = g.y - fj;
             const float a1 = (xlo_b - X0) * inv.x, a2 = (xhi_b - X0) * inv.x;
             const float b1 = (ylo_b - Y0) * inv.y, b2 = (yhi_b - Y0) * inv.y;
             const float lo = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), 0.0f);
             const float hi = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), (float)(N - 1));
-            if (!(lo <= hi)) continue;
             float wsum = 0.0f;
             const float kend = floorf(hi);
             for (float kf = ceilf(lo); kf <= kend; kf += 1.0f) {
@@ -126,7 +130,7 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
                 }
             }
 #pragma unroll
-            for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, load_f(sino[s], rbase + c), vsum[s]);
+            for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sv[s], vsum[s]);
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s];

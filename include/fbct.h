@@ -138,7 +138,9 @@ fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void
                        int n_slices, void *stream);
 
 /* Exact matched adjoint of fbct_fp (transpose of the same interpolation weights).
- * sino -> image.  No reference counterpart (the reference has no gradient). */
+ * sino -> image.  No reference counterpart (the reference has no gradient).  The
+ * sinogram must be finite: a NaN or infinity spreads to every pixel whose candidate
+ * rays include it, not only to the pixels the ray touches. */
 fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, int n_slices,
                             void *stream);
 
