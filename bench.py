@@ -25,8 +25,13 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "GUPS for FP & BP at 512²×984v; batched fwd+adjoint ms/slice vs plain"
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78,643 GB/s
+# LDS read bandwidth of one MI355X: 96.7 TB/s, measured with a ds_read_b128 throughput
+# probe (377 B/ns per CU over 256 CUs); AMD publishes no LDS figure for this part.
+LDS_PEAK_GBS = 96700.0
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of the dominant kernel measured with rocprofv3 counters (FETCH_SIZE and
+# WRITE_SIZE passes, calibrated on a plain copy); used when the workload matches.
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_clinical_64slices.json")
 
 
 def log(msg):
@@ -46,7 +51,7 @@ def main(argv=None):
                          "persistent launch per operator, autotuned; fdk: cone-beam FDK of a 512x512x256 "
                          "volume from 720 views of a 1024x256 flat panel, BF16 projections")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="dispatch each operator once on the batch (plus a 256 MiB calibration copy) "
+                    help="dispatch each operator once on the batch (plus a 256 MiB calibration kernel) "
                          "for counter collection under rocprofv3, then exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -84,18 +89,24 @@ def main(argv=None):
     torch.cuda.synchronize()
 
     if args.profile_kernels:
-        # calibration: a plain device copy of 256 MiB (reads and writes 262144 KiB)
+        # calibration: an elementwise kernel that reads 256 MiB and writes 256 MiB
+        # (262144 KiB each way), for scaling the FETCH_SIZE / WRITE_SIZE counters
         src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 1.5, out=dst)
         torch.cuda.synchronize()
         sino = target.contiguous()
+        xb16 = x.to(torch.bfloat16)
+        rb16 = (target - plan.fp(xb16)).to(torch.bfloat16)
+        torch.cuda.synchronize()
         for name, fn in (("fp", lambda: plan.fp(x)), ("fp_adjoint", lambda: plan.fp_adjoint(sino)),
+                         ("fp_bf16", lambda: plan.fp(xb16)), ("fp_adjoint_bf16", lambda: plan.fp_adjoint(rb16)),
                          ("bp", lambda: plan.bp(sino)), ("fbp", lambda: plan.fbp(sino))):
             fn()
             torch.cuda.synchronize()
             log("profile: %s dispatched" % name)
-        print(json.dumps({"profiled": ["copy 256MiB", "fp", "fp_adjoint", "bp", "fbp"], "slices": S,
+        print(json.dumps({"profiled": ["fill+copy 256MiB", "fp", "fp_adjoint", "fp_bf16", "fp_adjoint_bf16", "bp",
+                                       "fbp"], "slices": S,
                           "fp_samples_per_slice": samples}), flush=True)
         return
 
@@ -146,6 +157,7 @@ def main(argv=None):
     reps = max(3, args.steps // 2)
     log("bench: per-operator timings")
     sino = target.contiguous()
+    ms_fp_bf16 = time_op(lambda: plan.fp(xb), reps)  # the step's FP launch (dominant kernel)
     ms_fp = time_op(lambda: plan.fp(x), reps)
     ms_fpa = time_op(lambda: plan.fp_adjoint(sino), reps)
     ms_bp = time_op(lambda: plan.bp(sino), reps)
@@ -162,10 +174,19 @@ def main(argv=None):
 
     ms_plain = time_op(plain, max(2, reps // 2))
 
-    # dominant kernel: FP ray march (one launch per fp call)
-    fp_bytes = 4 * 4 * samples * S       # 4 taps x 4 B gathered per sample (algorithmic)
-    achieved = fp_bytes / (ms_fp * 1e-3) / 1e9
-    hbm_bytes = 4 * (nX * nY + nV * nC) * S  # compulsory: image in + sinogram out
+    # dominant kernel: the FP ray march of the step (BF16 images, one launch per fp call);
+    # its taps are gathered from an FP32 LDS tile: 4 taps x 4 B per sample and slice
+    fp_bytes = 4 * 4 * samples * S
+    achieved = fp_bytes / (ms_fp_bf16 * 1e-3) / 1e9
+    hbm_bytes = (2 * nX * nY + 4 * nV * nC) * S  # compulsory: BF16 image in + FP32 sinogram out
+    traffic = None
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("slices") == S and tr.get("workload") == "clinical":
+            traffic = tr["kernels"]["fp_kernel_bf16"]["hbm_bytes"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
     out = {
         "metric": METRIC,
         "value": round(gups, 3),
@@ -187,6 +208,7 @@ def main(argv=None):
         "ms_per_slice_batched_f32_storage": round(ms_step_f32 / S, 4),
         "gups_batched_f32_storage": round(2.0 * S * U / (ms_step_f32 * 1e-3) / 1e9, 3),
         "fp_gups": round(S * U / (ms_fp * 1e-3) / 1e9, 3),
+        "fp_bf16_gups": round(S * U / (ms_fp_bf16 * 1e-3) / 1e9, 3),
         "fp_adjoint_gups": round(S * U / (ms_fpa * 1e-3) / 1e9, 3),
         "bp_gups": round(S * U / (ms_bp * 1e-3) / 1e9, 3),
         "fbp_gups": round(S * U / (ms_fbp * 1e-3) / 1e9, 3),
@@ -196,17 +218,18 @@ def main(argv=None):
         "batched_f32_storage_vs_plain": round(ms_plain / ms_step_f32, 3),
         "wall_s": round(wall, 3),
         "roofline": {
-            "kernel": "fp_kernel",
+            "kernel": "fp_kernel<4, 8, bf16, float>",
             "bound": "lds",
             "achieved": round(achieved, 1),
             "peak": LDS_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / LDS_PEAK_GBS, 4),
-            "traffic": None,
-            "launch_ms": round(ms_fp, 4),
+            "traffic": traffic,
+            "launch_ms": round(ms_fp_bf16, 4),
             "algorithmic_bytes_per_launch": fp_bytes,
             "compulsory_hbm_bytes_per_launch": hbm_bytes,
-            "hbm_frac_compulsory": round(hbm_bytes / (ms_fp * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+            "hbm_frac_compulsory": round(hbm_bytes / (ms_fp_bf16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+            "peak_note": "LDS ds_read_b128 bandwidth measured on MI355X (no published figure)",
         },
     }
 

@@ -110,9 +110,11 @@ __device__ __forceinline__ int interp_pos(float xi, int nCols, float &al)
 template <bool kWeighted, int S, int kNc>
 __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
 {
-    const int i = blockIdx.x * kBpTx + threadIdx.x % kBpTx;
-    const int j = blockIdx.y * kBpTy + threadIdx.x / kBpTx;
-    const size_t sl = (size_t)slice0 + (size_t)S * blockIdx.z;
+    unsigned t, g;  // XCD-aware (tile, slice group)
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
+    const int i = (t % gridDim.x) * kBpTx + threadIdx.x % kBpTx;
+    const int j = (t / gridDim.x) * kBpTy + threadIdx.x / kBpTx;
+    const size_t sl = (size_t)slice0 + (size_t)S * g;
     if (i >= A.nX || j >= A.nY) return;
     const PixelGeo pg = pixel_geo(A, i, j);
     float acc[S];

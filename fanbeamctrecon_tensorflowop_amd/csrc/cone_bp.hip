@@ -56,10 +56,12 @@ __device__ __forceinline__ float fan_atan(const ConeArgs &A, float z)
 template <bool kFlat, int kNc, int KZ, bool kPairs, typename TO>
 __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
 {
-    const int i = blockIdx.x * kTx + threadIdx.x % kTx;
-    const int j = blockIdx.y * kTy + threadIdx.x / kTx;
-    const int zc = blockIdx.z % A.nzc;
-    const size_t vi = blockIdx.z / A.nzc;
+    unsigned t, gz;  // XCD-aware (tile, z-chunk of a volume): an XCD keeps whole z-chunks
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, gz);
+    const int i = (t % gridDim.x) * kTx + threadIdx.x % kTx;
+    const int j = (t / gridDim.x) * kTy + threadIdx.x / kTx;
+    const int zc = gz % A.nzc;
+    const size_t vi = gz / A.nzc;
     const int k0 = zc * KZ;
     if (i >= A.nX || j >= A.nY) return;
     const float x = fmaf(A.dx, (float)i + 0.5f, A.xmin);

@@ -21,6 +21,29 @@ constexpr int kFpNumCfg = 2;
 constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
 constexpr int kFpCfgSlab[kFpNumCfg] = {32, 20};
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each with its own
+// L2: the workgroup with linear id b runs on XCD b % 8.  xcd_block maps b to a logical
+// block id so that every XCD receives a contiguous range of logical blocks.
+constexpr unsigned kNumXcd = 8;
+
+__host__ __device__ inline unsigned xcd_block(unsigned b, unsigned total)
+{
+    const unsigned q = total / kNumXcd, r = total % kNumXcd;
+    const unsigned x = b % kNumXcd, k = b / kNumXcd;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// (tile, slice group) of workgroup b in a grid of `tiles` tiles x `groups` slice groups.
+// When the groups divide evenly over the XCDs, each XCD takes whole groups (their inputs
+// stay in its L2 and every XCD does the same work); otherwise the hardware order is kept.
+__host__ __device__ inline void xcd_tile(unsigned b, unsigned tiles, unsigned groups, unsigned &tile,
+                                         unsigned &group)
+{
+    const unsigned l = groups % kNumXcd == 0 ? xcd_block(b, tiles * groups) : b;
+    group = l / tiles;
+    tile = l % tiles;
+}
+
 // BP atan(z) = z + z^3 R(z^2): at most this many coefficients of R.
 constexpr int kBpAtanMax = 9;
 

@@ -259,7 +259,9 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 template <int S, int G, typename TI, typename TO>
 __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
 {
-    fp_bundle<S, G, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S, err);
+    unsigned t, g;  // XCD-aware (bundle + view tile, slice group)
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
+    fp_bundle<S, G, TI, TO>(A, t % gridDim.x, t / gridDim.x, g * S, err);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, view,
