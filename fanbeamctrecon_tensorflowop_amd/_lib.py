@@ -71,7 +71,7 @@ class ConeGeom(ctypes.Structure):
         ("rowBottom", ctypes.c_float), ("dShift", ctypes.c_float), ("nX", ctypes.c_int32),
         ("nY", ctypes.c_int32), ("nZ", ctypes.c_int32), ("dx", ctypes.c_float), ("dy", ctypes.c_float),
         ("dz", ctypes.c_float), ("xmin", ctypes.c_float), ("ymin", ctypes.c_float), ("zmin", ctypes.c_float),
-        ("rFOV", ctypes.c_float),
+        ("rFOV", ctypes.c_float), ("short_scan", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -94,6 +94,8 @@ SIGNATURES = [
     ("fbct_plan_ray_table", _I, [_P, ctypes.POINTER(ctypes.c_int32), _F]),
     ("fbct_plan_ramp_taps", _I, [_P, ctypes.POINTER(ctypes.c_double)]),
     ("fbct_plan_upload", _I, [_P]),
+    ("fbct_plan_parker_weights", _I, [_P, _F]),
+    ("fbct_fbp_short_scan", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fp", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fp_adjoint", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fp_ex", _I, [_P, _P, _I, _P, _I, _I, _P]),

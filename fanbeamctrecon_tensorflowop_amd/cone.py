@@ -24,10 +24,12 @@ _DETECTOR = {"flat": 0, "curved": 1}
 
 
 def cone_geom(*, R, D, n_views, d_angle, detector, n_cols, n_rows, d_col, d_row, col_left=None, row_bottom=None,
-              d_shift=0.0, nx, ny=None, nz, dx, dy=None, dz, xmin=None, ymin=None, zmin=None, rfov=None) -> dict:
+              d_shift=0.0, nx, ny=None, nz, dx, dy=None, dz, xmin=None, ymin=None, zmin=None, rfov=None,
+              short_scan=False) -> dict:
     """Geometry dict (the fields of ``fbct_cone_geom``); centred detector and volume by
     default (colLeft = -nCols dCol / 2, rowBottom = -nRows dRow / 2, corners at -n d / 2,
-    rFOV = nX dx / 2)."""
+    rFOV = nX dx / 2).  short_scan: Parker-weighted reconstruction of a scan over 180
+    degrees plus the fan angle."""
     ny = nx if ny is None else ny
     dy = dx if dy is None else dy
     g = {
@@ -38,6 +40,7 @@ def cone_geom(*, R, D, n_views, d_angle, detector, n_cols, n_rows, d_col, d_row,
         "dShift": d_shift, "nX": nx, "nY": ny, "nZ": nz, "dx": dx, "dy": dy, "dz": dz,
         "xmin": -0.5 * nx * dx if xmin is None else xmin, "ymin": -0.5 * ny * dy if ymin is None else ymin,
         "zmin": -0.5 * nz * dz if zmin is None else zmin, "rFOV": 0.5 * nx * dx if rfov is None else rfov,
+        "short_scan": 1 if short_scan else 0,
     }
     # the values the library receives (float32 fields)
     st = _lib.ConeGeom(**g)

@@ -192,6 +192,30 @@ fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, i
     return filter_rows(*plan, *t, q, sino, (int64_t)n_slices * plan->g.nViews, FBCT_FILTER_MFMA, st);
 }
 
+fbct_status fbct_fbp_short_scan(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
+                                size_t workspace_bytes, void *stream)
+{
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp_short_scan");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
+        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_short_scan: workspace of %zu bytes needed, got %zu",
+                    sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    const float *pk = nullptr;
+    if ((s = fbct::device_parker(plan, t, &pk)) != FBCT_OK) return s;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float *q = static_cast<float *>(workspace);
+    fbct::RampOperand H{t->ramp_planes, plan->g.nCols, plan->d.ramp_np, plan->d.ramp_kp};
+    fbct::RowWeights rw;
+    rw.w = pk;  // row r = slice * nViews + view
+    rw.wrows = plan->g.nViews;
+    if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, (int64_t)n_slices * plan->g.nViews, rw, st, false)) !=
+        FBCT_OK)
+        return s;
+    return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);
+}
+
 fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices, void *workspace,
                      size_t workspace_bytes, void *stream)
 {

@@ -47,6 +47,8 @@ fbct_status validate(const fbct_cone_geom &g)
         return fail(FBCT_ERR_INVALID_GEOMETRY, "cone: voxel sizes must be positive");
     for (float v : {g.dAngle, g.colLeft, g.rowBottom, g.dShift, g.xmin, g.ymin, g.zmin})
         if (!std::isfinite(v)) return fail(FBCT_ERR_INVALID_GEOMETRY, "cone: non-finite geometry value");
+    if (g.short_scan != 0 && g.short_scan != 1)
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "cone: short_scan must be 0 or 1 (got %d)", g.short_scan);
     if (g.detector == FBCT_DETECTOR_CURVED) {
         const double a0 = (double)g.colLeft - (double)g.dShift * g.dCol;
         const double a1 = (double)g.colLeft + ((double)g.nCols - g.dShift) * g.dCol;
@@ -54,6 +56,21 @@ fbct_status validate(const fbct_cone_geom &g)
             return fail(FBCT_ERR_INVALID_GEOMETRY, "cone: curved detector fan must stay within +-90 degrees");
     }
     return FBCT_OK;
+}
+
+// Fan angle of detector coordinate xi (columns) as the backprojection maps it.
+double cone_gamma(const fbct_cone_geom &g, double xi)
+{
+    const double u = (double)g.colLeft + (xi - (double)g.dShift) * (double)g.dCol;
+    return g.detector == FBCT_DETECTOR_FLAT ? std::atan(u / (double)g.D) : u;
+}
+
+bool build_parker(fbct_cone_plan &p)
+{
+    std::vector<double> gamma(p.g.nCols);
+    for (int c = 0; c < p.g.nCols; ++c) gamma[c] = cone_gamma(p.g, c + 0.5);
+    const double delta = std::max(std::fabs(cone_gamma(p.g, 0.0)), std::fabs(cone_gamma(p.g, (double)p.g.nCols)));
+    return parker_weights(p.angles, gamma, delta, p.parker);
 }
 
 void build(fbct_cone_plan &p)
@@ -138,6 +155,7 @@ void free_cone_tables(ConeDeviceTables &t)
     (void)hipFree(t.view_csd);
     (void)hipFree(t.preweight);
     (void)hipFree(t.ramp_planes);
+    (void)hipFree(t.parker);
     if (cur >= 0) (void)hipSetDevice(cur);
     t = ConeDeviceTables{};
 }
@@ -165,6 +183,7 @@ fbct_status cone_tables(fbct_cone_plan *p, const ConeDeviceTables **out)
     if (s == FBCT_OK) s = upload_vec(&t.view_csd, csd);
     if (s == FBCT_OK && !p->preweight.empty()) s = upload_vec(&t.preweight, p->preweight);
     if (s == FBCT_OK) s = upload_vec(&t.ramp_planes, p->ramp_planes);
+    if (s == FBCT_OK && !p->parker.empty()) s = upload_vec(&t.parker, p->parker);
     if (s != FBCT_OK) {
         free_cone_tables(t);
         return s;
@@ -205,7 +224,13 @@ fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void 
 {
     RampOperand H{t.ramp_planes, p->g.nCols, p->d.ramp_np, p->d.ramp_kp};
     const int64_t rows = (int64_t)n * p->g.nViews * p->g.nRows;
-    return launch_ramp_gemm(H, proj, ti, q, rows, t.preweight, p->g.nRows, st, pairs);
+    RowWeights rw;
+    rw.w = t.preweight;  // [nRows][nCols] cosine weights (flat detector)
+    rw.wrows = p->g.nRows;
+    rw.w2 = t.parker;    // [nViews][nCols] Parker weights (short scan)
+    rw.w2div = p->g.nRows;
+    rw.w2rows = p->g.nViews;
+    return launch_ramp_gemm(H, proj, ti, q, rows, rw, st, pairs);
 }
 
 }  // namespace
@@ -268,6 +293,10 @@ fbct_status fbct_cone_plan_create(const fbct_cone_geom *geom, const float *angle
         p->g = *geom;
         p->angles.assign(angles, angles + n_angles);
         fbct::build(*p);
+        if (geom->short_scan && !fbct::build_parker(*p)) {
+            delete p;
+            return fail(FBCT_ERR_INVALID_GEOMETRY, "cone: short scan views do not cover 180 degrees plus the fan angle");
+        }
     } catch (const std::bad_alloc &) {
         delete p;
         return fail(FBCT_ERR_OUT_OF_MEMORY, "cone plan tables");

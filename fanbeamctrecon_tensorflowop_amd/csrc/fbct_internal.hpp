@@ -92,6 +92,7 @@ struct DeviceTables {
     double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
     double *ramp = nullptr;        // [2*nCols-1]
     uint16_t *ramp_planes = nullptr;  // [3][Np][Kp] BF16 split planes of the Toeplitz matrix H
+    float *parker = nullptr;       // [nViews][nCols] short-scan weights (uploaded on first use)
 };
 
 // Scalars the kernels need, derived once in FP64 on the host.
@@ -172,6 +173,7 @@ struct ConeDeviceTables {
     double2 *view_csd = nullptr;   // [nViews]
     float *preweight = nullptr;    // [nRows][nCols] (flat detector)
     uint16_t *ramp_planes = nullptr;  // [3][Np][Kp]
+    float *parker = nullptr;       // [nViews][nCols] short-scan weights (short_scan plans)
 };
 
 }  // namespace fbct
@@ -184,6 +186,7 @@ struct fbct_cone_plan {
     std::vector<float> preweight;  // empty for the curved detector
     std::vector<double> taps;      // 2*nCols-1, scaled
     std::vector<uint16_t> ramp_planes;
+    std::vector<float> parker;     // [nViews][nCols], short_scan plans only
     fbct::ConeDerived d{};
     std::mutex mu;
     std::vector<fbct::ConeDeviceTables> dev;
@@ -230,8 +233,17 @@ fbct_status upload_vec(T **dst, const std::vector<T> &src)
     return FBCT_OK;
 }
 
+// Parker short-scan weights w[v][c] (Parker 1982, smooth sin^2 transitions) times 2, for
+// view angles `angles` and column fan angles gamma[c] (the angles the backprojection
+// assigns), delta = the largest |fan angle| of the detector edges.  False if the views do
+// not cover pi + 2 delta.
+bool parker_weights(const std::vector<float> &angles, const std::vector<double> &gamma, double delta,
+                    std::vector<float> &w);
+
 // Get (uploading if needed) the tables for the current device.
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
+// The plan's Parker weights on the current device (computed and uploaded on first use).
+fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **out);
 
 // Kernel launchers (fp.hip, bp.hip, filter.hip).  All asynchronous on `stream`.
 fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti,
@@ -255,8 +267,14 @@ struct RampOperand {
 };
 // rows x nC inputs (FP32 or BF16 storage), optionally times w[r % wrows][c], filtered
 // into FP32 outputs; pairs: written as (q[c], q[min(c+1, nC-1)]) float2 per column.
+struct RowWeights {
+    const float *w = nullptr;   // row r uses w[r % wrows][c]
+    int wrows = 1;
+    const float *w2 = nullptr;  // and w2[(r / w2div) % w2rows][c]
+    int w2div = 1, w2rows = 1;
+};
 fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
-                             const float *w, int wrows, hipStream_t stream, bool pairs);
+                             const RowWeights &rw, hipStream_t stream, bool pairs);
 // q: filtered projections, plain [..][nCols] FP32 or (pairs) the column-pair layout.
 fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, bool pairs,
                            void *vol, fbct_dtype to, int n, hipStream_t stream);

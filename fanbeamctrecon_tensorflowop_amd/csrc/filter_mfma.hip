@@ -37,8 +37,9 @@ struct MfmaArgs {
     float *out;        // [rows][nC]
     const __bf16 *hp;  // [3][Np][Kp] split planes of H
     const float *w;    // optional per-sample weights [wrows][nC], row r uses w[r % wrows]
+    const float *w2;   // optional second factor [w2rows][nC], row r uses w2[(r / w2div) % w2rows]
     long long rows;
-    int nC, Np, Kp, wrows;
+    int nC, Np, Kp, wrows, w2div, w2rows;
 };
 
 __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &ll)
@@ -70,7 +71,8 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
     const long long arow = r0 + (lane & 15);
     const bool arow_ok = arow < A.rows;
     const TI *prow = static_cast<const TI *>(A.in) + (arow_ok ? arow : 0) * (long long)A.nC;
-    const float *wrow = kW ? A.w + (arow_ok ? arow % A.wrows : 0) * (long long)A.nC : nullptr;
+    const float *wrow = kW && A.w ? A.w + (arow_ok ? arow % A.wrows : 0) * (long long)A.nC : nullptr;
+    const float *w2row = kW && A.w2 ? A.w2 + (arow_ok ? (arow / A.w2div) % A.w2rows : 0) * (long long)A.nC : nullptr;
     const int kl = 8 * (lane >> 4);  // this lane's k offset within a 32-wide step
     const size_t plane = (size_t)A.Np * A.Kp;
 
@@ -103,7 +105,13 @@ __global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
         }
         if constexpr (kW) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] *= (arow_ok && kb + j < A.nC) ? wrow[kb + j] : 0.0f;
+            for (int j = 0; j < 8; ++j) {
+                const bool ok = arow_ok && kb + j < A.nC;
+                float wt = ok ? 1.0f : 0.0f;
+                if (wrow) wt *= ok ? wrow[kb + j] : 0.0f;
+                if (w2row) wt *= ok ? w2row[kb + j] : 0.0f;
+                x[j] *= wt;
+            }
         }
         bf16x8 ah, al, all;
 #pragma unroll
@@ -172,20 +180,25 @@ void launch_mfma_p(bool pairs, bool aligned, dim3 grid, hipStream_t stream, cons
 }  // namespace
 
 fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
-                             const float *w, int wrows, hipStream_t stream, bool pairs)
+                             const RowWeights &rw, hipStream_t stream, bool pairs)
 {
     if (rows <= 0) return FBCT_OK;
+    if ((rw.w && rw.wrows <= 0) || (rw.w2 && (rw.w2div <= 0 || rw.w2rows <= 0)))
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "ramp filter: weight row counts must be positive");
+    const float *w = (rw.w || rw.w2) ? (rw.w ? rw.w : rw.w2) : nullptr;  // non-null: weighted kernel
     MfmaArgs A;
     A.in = in;
     A.out = out;
     A.hp = reinterpret_cast<const __bf16 *>(H.planes);
-    A.w = w;
+    A.w = rw.w;
+    A.w2 = rw.w2;
     A.rows = rows;
     A.nC = H.nC;
     A.Np = H.Np;
     A.Kp = H.Kp;
-    A.wrows = w ? wrows : 1;
-    if (w && wrows <= 0) return fail(FBCT_ERR_INVALID_ARGUMENT, "ramp filter: weight rows must be positive");
+    A.wrows = rw.w ? rw.wrows : 1;
+    A.w2div = rw.w2 ? rw.w2div : 1;
+    A.w2rows = rw.w2 ? rw.w2rows : 1;
     const long long gx = (rows + 63) / 64;
     if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
     dim3 grid((unsigned)gx, (H.nC + kWaveCols - 1) / kWaveCols);
@@ -207,7 +220,7 @@ fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, c
                                     int64_t rows, hipStream_t stream)
 {
     RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp};
-    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, nullptr, 1, stream, false);
+    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, RowWeights{}, stream, false);
 }
 
 }  // namespace fbct

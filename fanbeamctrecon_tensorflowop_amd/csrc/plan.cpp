@@ -156,6 +156,48 @@ double bf16_value(uint16_t b)
     return static_cast<double>(f);
 }
 
+bool parker_weights(const std::vector<float> &angles, const std::vector<double> &gamma, double delta,
+                    std::vector<float> &w)
+{
+    const double pi = 3.14159265358979323846;
+    const int nV = static_cast<int>(angles.size()), nC = static_cast<int>(gamma.size());
+    if (nV < 1) return false;
+    const double s = angles[nV - 1] >= angles[0] ? 1.0 : -1.0;
+    const double span = s * (static_cast<double>(angles[nV - 1]) - angles[0]);
+    const double step = nV > 1 ? span / (nV - 1) : 0.0;  // the last view may end one step short
+    if (span + step < pi + 2.0 * delta - 1e-9) return false;
+    w.assign(static_cast<size_t>(nV) * nC, 0.0f);
+    for (int v = 0; v < nV; ++v) {
+        const double b = s * (static_cast<double>(angles[v]) - angles[0]);
+        for (int c = 0; c < nC; ++c) {
+            const double g = s * gamma[c];
+            double x;
+            if (b <= 2.0 * delta - 2.0 * g) {
+                const double t = delta - g > 0.0 ? std::sin(0.25 * pi * b / (delta - g)) : 1.0;
+                x = t * t;
+            } else if (b <= pi - 2.0 * g) {
+                x = 1.0;
+            } else if (b <= pi + 2.0 * delta) {
+                const double t = delta + g > 0.0 ? std::sin(0.25 * pi * (pi + 2.0 * delta - b) / (delta + g)) : 0.0;
+                x = t * t;
+            } else {
+                x = 0.0;
+            }
+            w[static_cast<size_t>(v) * nC + c] = static_cast<float>(2.0 * x);
+        }
+    }
+    return true;
+}
+
+// Fan angles of the column centres as the BP maps them, and the detector edge bound.
+static void fan_gammas(const fbct_fan_geom &g, std::vector<double> &gamma, double &delta)
+{
+    const double dl = g.dLeft, dc = g.dCol, sh = g.dShift;
+    gamma.resize(g.nCols);
+    for (int c = 0; c < g.nCols; ++c) gamma[c] = dl + (c + 0.5 - sh) * dc;
+    delta = std::max(std::fabs(dl - sh * dc), std::fabs(dl + (g.nCols - sh) * dc));
+}
+
 void toeplitz_planes(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &planes)
 {
     planes.assign(static_cast<size_t>(3) * np * kp, 0);
@@ -394,8 +436,30 @@ static void free_tables(DeviceTables &t)
     (void)hipFree(t.view_csd);
     (void)hipFree(t.ramp);
     (void)hipFree(t.ramp_planes);
+    (void)hipFree(t.parker);
     if (cur >= 0) (void)hipSetDevice(cur);
     t = DeviceTables{};
+}
+
+fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **out)
+{
+    std::lock_guard<std::mutex> lk(plan->mu);
+    DeviceTables *dt = nullptr;
+    for (DeviceTables &d : plan->dev)
+        if (&d == t) dt = &d;
+    if (!dt) return fail(FBCT_ERR_DEVICE, "short scan: tables not resident on this device");
+    if (!dt->parker) {
+        std::vector<double> gamma;
+        double delta;
+        fan_gammas(plan->g, gamma, delta);
+        std::vector<float> w;
+        if (!parker_weights(plan->angles, gamma, delta, w))
+            return fail(FBCT_ERR_INVALID_GEOMETRY, "short scan: the views do not cover 180 degrees plus the fan angle");
+        fbct_status s = upload_vec(&dt->parker, w);
+        if (s != FBCT_OK) return s;
+    }
+    *out = dt->parker;
+    return FBCT_OK;
 }
 
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
@@ -536,6 +600,19 @@ fbct_status fbct_plan_ramp_taps(const fbct_plan *plan, double *taps_out)
 {
     if (!plan || !taps_out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null pointer");
     std::memcpy(taps_out, plan->ramp.data(), plan->ramp.size() * sizeof(double));
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_parker_weights(const fbct_plan *plan, float *w_out)
+{
+    if (!plan || !w_out) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null pointer");
+    std::vector<double> gamma;
+    double delta;
+    fbct::fan_gammas(plan->g, gamma, delta);
+    std::vector<float> w;
+    if (!fbct::parker_weights(plan->angles, gamma, delta, w))
+        return fbct::fail(FBCT_ERR_INVALID_GEOMETRY, "short scan: the views do not cover 180 degrees plus the fan angle");
+    std::memcpy(w_out, w.data(), w.size() * sizeof(float));
     return FBCT_OK;
 }
 

@@ -187,6 +187,17 @@ class FanBeamPlan:
         return self._run("fbct_fbp_adjoint", self._prep(image, self.image_shape, "fbp_adjoint"),
                          self.sino_shape, extra_ws=True)
 
+    def parker_weights(self) -> np.ndarray:
+        """Short-scan weights [nViews, nCols] (Parker's weights times 2)."""
+        w = np.zeros((self.n_views, self.n_cols), dtype=np.float32)
+        _lib.check(_lib.load().fbct_plan_parker_weights(self._h, _fptr(w)), "fbct_plan_parker_weights")
+        return w
+
+    def fbp_short_scan(self, sino: torch.Tensor) -> torch.Tensor:
+        """FBP of a short scan (180 degrees plus the fan angle) with Parker's weights."""
+        return self._run("fbct_fbp_short_scan", self._prep(sino, self.sino_shape, "fbp_short_scan"),
+                         self.image_shape, extra_ws=True)
+
     def fdk(self, proj: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_fdk", self._prep(proj, self.sino_shape, "fdk"), self.image_shape, extra_ws=True)
 

@@ -98,22 +98,30 @@ def poisson_target(line_integrals, photons=1e5, seed=7):
 
 
 def cone_scan(*, nx=512, nz=256, dx=0.7, dz=None, n_views=720, n_cols=1024, n_rows=256, pitch=1.0,
-              row_pitch=None, R=541.0, D=949.0, detector="flat", n_views_range=None):
+              row_pitch=None, R=541.0, D=949.0, detector="flat", short_scan=False):
     """(geometry dict of fbct_cone_geom, angles) for a circular orbit: full 360 degree
     scan of n_views views, a centred detector of n_cols x n_rows cells (pitch mm for the
     flat detector at distance D; the curved one spans the same fan), and a centred
     nx x nx x nz volume of dx x dx x dz voxels (FOV radius nx dx / 2).  The default is
-    the 512 x 512 x 256 volume, 720 views, 1024 x 256 flat panel at 1.0 mm workload."""
+    the 512 x 512 x 256 volume, 720 views, 1024 x 256 flat panel at 1.0 mm workload.
+    short_scan: the views span 180 degrees plus the fan angle (plus one view spacing) at
+    the full-scan spacing of n_views views."""
     from .cone import cone_geom
     dz = dx if dz is None else dz
     row_pitch = pitch if row_pitch is None else row_pitch
     if detector == "flat":
         d_col = pitch
+        half_fan = math.atan(0.5 * n_cols * pitch / D)
     else:
         d_col = 2.0 * math.atan(0.5 * n_cols * pitch / D) / n_cols
-    g = cone_geom(R=R, D=D, n_views=n_views, d_angle=2.0 * math.pi / n_views, detector=detector, n_cols=n_cols,
-                  n_rows=n_rows, d_col=d_col, d_row=row_pitch, nx=nx, nz=nz, dx=dx, dz=dz)
-    return g, angle(n_views, 0.0)
+        half_fan = 0.5 * n_cols * d_col
+    step = 2.0 * math.pi / n_views
+    angles = angle(n_views, 0.0)
+    if short_scan:
+        angles = angles[:int(math.ceil((math.pi + 2.0 * half_fan) / step)) + 1]
+    g = cone_geom(R=R, D=D, n_views=len(angles), d_angle=step, detector=detector, n_cols=n_cols,
+                  n_rows=n_rows, d_col=d_col, d_row=row_pitch, nx=nx, nz=nz, dx=dx, dz=dz, short_scan=short_scan)
+    return g, angles
 
 
 def random_ellipsoids(n=10, seed=5, rfov=179.2, zhalf=89.6, vmax=0.04):

@@ -123,6 +123,12 @@ fbct_status fbct_plan_ray_table(const fbct_plan *plan, int32_t *n_out, float *ge
 /* Host copy of the ramp taps h[n], n in [-(nCols-1), nCols-1] (2*nCols-1 doubles). */
 fbct_status fbct_plan_ramp_taps(const fbct_plan *plan, double *taps_out);
 
+/* Host copy of the short-scan weights w[nViews][nCols] used by fbct_fbp_short_scan:
+ * Parker's weights (Parker 1982) times 2.  FBCT_ERR_INVALID_GEOMETRY if the views do not
+ * cover 180 degrees plus the fan angle.  No reference implementation: config.py:23-24
+ * declares ParkWeighting and FDK/GPURecon_GPU.h:53 comments out ApplyParkerWeighting. */
+fbct_status fbct_plan_parker_weights(const fbct_plan *plan, float *w_out);
+
 /* ------------------------------------------------------------------- device --
  * Upload the plan's tables to the current HIP device (done implicitly by the first
  * execute on a device; call it before capturing a HIP graph). */
@@ -188,6 +194,13 @@ fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, fl
 /* Exact adjoint of fbct_fbp.  image -> sino.  No reference counterpart. */
 fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
                              void *workspace, size_t workspace_bytes, void *stream);
+
+/* Short-scan FBP: the sinogram is multiplied by fbct_plan_parker_weights before the
+ * ramp filter (one MFMA launch), then backprojected as fbct_fbp.  For a scan over
+ * 180 degrees plus the fan angle (parameter[3] gives the angular range).  The first call
+ * on a device uploads the weights. */
+fbct_status fbct_fbp_short_scan(fbct_plan *plan, const float *sino, float *image, int n_slices,
+                                void *workspace, size_t workspace_bytes, void *stream);
 
 /* The reference FDK op ("BackProjection", FDK/FBP.cc:16-155 -> BP_One_Rot_3D_GPU,
  * FDK/GPURecon_GPU.cu:163-268; kernel :15-50).  The reference pins nZ = nRows = 1
@@ -285,6 +298,9 @@ typedef struct fbct_cone_geom {
     float dx, dy, dz;     /* voxel size (mm)                                         */
     float xmin, ymin, zmin; /* volume corner (mm)                                    */
     float rFOV;           /* reconstructed cylinder radius (mm)                      */
+    int32_t short_scan;   /* 1: Parker short-scan weights (per view and column, the    */
+                          /*    fan angle atan(u/D) for the flat detector) before the  */
+                          /*    filter; the views must cover 180 deg plus the fan      */
 } fbct_cone_geom;
 
 typedef struct fbct_cone_plan fbct_cone_plan;

@@ -566,33 +566,62 @@ typedef struct {
     const float *angles;
     const double *in;
     double *out;
-    const double *h, *w;
+    const double *h, *w, *pk;
 } orc_cone_ctx;
 
 static void cone_filt_item(void *vctx, long long line)
 {
     const orc_cone_ctx *x = (const orc_cone_ctx *)vctx;
     const int nC = x->g->nCols, r = (int)(line % x->g->nRows);
+    const int v = (int)((line / x->g->nRows) % x->g->nViews);
     const double *p = x->in + line * (long long)nC;
     const double *w = x->w + (long long)r * nC;
+    const double *pk = x->pk ? x->pk + (long long)v * nC : NULL;
     double *q = x->out + line * (long long)nC;
     for (int c = 0; c < nC; ++c) {
         double acc = 0.0;
-        for (int m = 0; m < nC; ++m) acc += p[m] * w[m] * x->h[c - m + nC - 1];
+        for (int m = 0; m < nC; ++m) acc += p[m] * w[m] * (pk ? pk[m] : 1.0) * x->h[c - m + nC - 1];
         q[c] = acc;
     }
 }
 
-void orc_cone_filter(const orc_cone *g, const double *proj, double *q, int nS, int nthreads)
+/* Fan angle of column c's centre as the cone backprojection maps it. */
+static double cone_gamma(const orc_cone *g, double xi)
 {
+    const double u = g->colLeft + (xi - g->dShift) * g->dCol;
+    return g->detector == 0 ? atan(u / g->D) : u;
+}
+
+static int cone_parker(const orc_cone *g, const float *angles, double *pk)
+{
+    double *gam = (double *)malloc(sizeof(double) * g->nCols);
+    for (int c = 0; c < g->nCols; ++c) gam[c] = cone_gamma(g, c + 0.5);
+    const double delta = fmax(fabs(cone_gamma(g, 0.0)), fabs(cone_gamma(g, (double)g->nCols)));
+    const int rc = orc_parker_weights(angles, g->nViews, gam, g->nCols, delta, pk);
+    free(gam);
+    return rc;
+}
+
+int orc_cone_filter(const orc_cone *g, const float *angles, const double *proj, double *q, int nS, int nthreads)
+{
+    double *pk = NULL;
+    if (g->short_scan) {
+        pk = (double *)malloc(sizeof(double) * (size_t)g->nViews * g->nCols);
+        if (cone_parker(g, angles, pk) != 0) {
+            free(pk);
+            return -1;
+        }
+    }
     double *h = (double *)malloc(sizeof(double) * (2 * g->nCols - 1));
     double *w = (double *)malloc(sizeof(double) * (size_t)g->nRows * g->nCols);
     orc_cone_taps(g, h);
     orc_cone_preweight(g, w);
-    orc_cone_ctx x = {g, NULL, proj, q, h, w};
+    orc_cone_ctx x = {g, NULL, proj, q, h, w, pk};
     orc_parallel_for((long long)nS * g->nViews * g->nRows, nthreads, cone_filt_item, &x);
     free(h);
     free(w);
+    free(pk);
+    return 0;
 }
 
 static void cone_bp_item(void *vctx, long long item)
@@ -637,15 +666,80 @@ static void cone_bp_item(void *vctx, long long item)
 
 void orc_cone_bp(const orc_cone *g, const float *angles, const double *q, double *vol, int nS, int nthreads)
 {
-    orc_cone_ctx x = {g, angles, q, vol, NULL, NULL};
+    orc_cone_ctx x = {g, angles, q, vol, NULL, NULL, NULL};
     orc_parallel_for((long long)nS * g->nY, nthreads, cone_bp_item, &x);
 }
 
-void orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS, int nthreads)
+int orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS, int nthreads)
 {
     const long long n = (long long)nS * g->nViews * g->nRows * g->nCols;
     double *q = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
-    orc_cone_filter(g, proj, q, nS, nthreads);
+    if (orc_cone_filter(g, angles, proj, q, nS, nthreads) != 0) {
+        free(q);
+        return -1;
+    }
     orc_cone_bp(g, angles, q, vol, nS, nthreads);
     free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------ short scan */
+
+int orc_parker_weights(const float *angles, int nViews, const double *gamma, int nCols, double delta, double *w)
+{
+    const double pi = 3.14159265358979323846;
+    const double s = angles[nViews - 1] >= angles[0] ? 1.0 : -1.0;
+    const double span = s * ((double)angles[nViews - 1] - (double)angles[0]);
+    /* the last view may sit one view spacing short of the end of the range */
+    const double step = nViews > 1 ? span / (nViews - 1) : 0.0;
+    if (span + step < pi + 2.0 * delta - 1e-9) return -1;
+    for (int v = 0; v < nViews; ++v) {
+        const double b = s * ((double)angles[v] - (double)angles[0]);
+        for (int c = 0; c < nCols; ++c) {
+            const double g = s * gamma[c];
+            double x;
+            if (b <= 2.0 * delta - 2.0 * g) {
+                const double t = delta - g > 0.0 ? sin(0.25 * pi * b / (delta - g)) : 1.0;
+                x = t * t;
+            } else if (b <= pi - 2.0 * g) {
+                x = 1.0;
+            } else if (b <= pi + 2.0 * delta) {
+                const double t = delta + g > 0.0 ? sin(0.25 * pi * (pi + 2.0 * delta - b) / (delta + g)) : 0.0;
+                x = t * t;
+            } else {
+                x = 0.0;
+            }
+            w[(long long)v * nCols + c] = 2.0 * x;
+        }
+    }
+    return 0;
+}
+
+int orc_parker_fan(const orc_geom *g, const float *angles, double *w)
+{
+    double *gam = (double *)malloc(sizeof(double) * g->nCols);
+    for (int c = 0; c < g->nCols; ++c)
+        gam[c] = (double)g->dLeft + ((double)c + 0.5 - (double)g->dShift) * (double)g->dCol;
+    const double e0 = (double)g->dLeft - (double)g->dShift * (double)g->dCol;
+    const double e1 = (double)g->dLeft + ((double)g->nCols - (double)g->dShift) * (double)g->dCol;
+    const int rc = orc_parker_weights(angles, g->nViews, gam, g->nCols, fmax(fabs(e0), fabs(e1)), w);
+    free(gam);
+    return rc;
+}
+
+int orc_fbp_short_scan(const orc_geom *g, const float *angles, const double *sino, double *img, int nS,
+                       int nthreads)
+{
+    const long long per = (long long)g->nViews * g->nCols;
+    double *w = (double *)malloc(sizeof(double) * per);
+    if (orc_parker_fan(g, angles, w) != 0) {
+        free(w);
+        return -1;
+    }
+    double *p = (double *)malloc(sizeof(double) * (nS * per > 0 ? nS * per : 1));
+    for (long long k = 0; k < nS * per; ++k) p[k] = sino[k] * w[k % per];
+    orc_fbp(g, angles, p, img, nS, nthreads);
+    free(p);
+    free(w);
+    return 0;
 }

@@ -140,6 +140,7 @@ typedef struct orc_cone {
     double dx, dy, dz;
     double xmin, ymin, zmin;
     double rFOV;
+    int short_scan;       /* 1: Parker short-scan weights on the projections          */
 } orc_cone;
 
 /* Filter taps h[n + nCols - 1], n in [-(nCols-1), nCols-1], with the constant scale. */
@@ -148,16 +149,41 @@ void orc_cone_taps(const orc_cone *g, double *h);
 /* Pre-weight w[r][c] (1 for the curved detector). */
 void orc_cone_preweight(const orc_cone *g, double *w);
 
-/* proj[nS][nViews][nRows][nCols] -> filtered q (same layout). */
-void orc_cone_filter(const orc_cone *g, const double *proj, double *q, int nS, int nthreads);
+/* proj[nS][nViews][nRows][nCols] -> filtered q (same layout); with short_scan the
+ * projections are also multiplied by Parker's weights (orc_parker_weights; the fan angle
+ * of a column is atan(u/D) for the flat detector).  Returns -1 if the scan is too short. */
+int orc_cone_filter(const orc_cone *g, const float *angles, const double *proj, double *q, int nS, int nthreads);
 
 /* Backprojection of filtered q -> vol[nS][nZ][nY][nX]. */
 void orc_cone_bp(const orc_cone *g, const float *angles, const double *q, double *vol, int nS,
                  int nthreads);
 
-/* FDK = orc_cone_filter + orc_cone_bp. */
-void orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS,
-                  int nthreads);
+/* FDK = orc_cone_filter + orc_cone_bp.  Returns -1 if a short scan is too short. */
+int orc_cone_fdk(const orc_cone *g, const float *angles, const double *proj, double *vol, int nS,
+                 int nthreads);
+
+/* ------------------------------------------------------------ short scan --
+ * Parker's weights (Parker 1982, smooth sin^2 transitions) for a scan over
+ * [0, pi + 2 delta] (no reference implementation: config.py:23-24 declares
+ * ParkWeighting, FDK/GPURecon_GPU.h:53 comments out ApplyParkerWeighting).  With
+ * b = s (angles[v] - angles[0]) and g = s gamma[c] (s = sign of the rotation):
+ *   w = sin^2(pi/4 b / (delta - g))               for 0 <= b <= 2 delta - 2 g
+ *     = 1                                         up to b = pi - 2 g
+ *     = sin^2(pi/4 (pi + 2 delta - b)/(delta + g)) up to b = pi + 2 delta, 0 beyond,
+ * times 2: the conjugate ray of (b, g) is (b + pi + 2 g, -g) and their weights sum to
+ * one, where a full scan counts every ray twice (the factor 1/2 of the filter taps).
+ * gamma[c] is the fan angle of column c's centre as the backprojection maps it;
+ * delta = max |fan angle| of the detector edges.  Returns -1 if the views do not
+ * cover pi + 2 delta. */
+int orc_parker_weights(const float *angles, int nViews, const double *gamma, int nCols, double delta, double *w);
+
+/* Parker weights of a fan geometry, w[nViews][nCols]: gamma = dLeft + (c + 1/2 - dShift) dCol. */
+int orc_parker_fan(const orc_geom *g, const float *angles, double *w);
+
+/* Short-scan FBP: filter of the Parker-weighted sinogram + 1/L^2-weighted BP.  Returns
+ * -1 (and leaves img untouched) if the scan is too short. */
+int orc_fbp_short_scan(const orc_geom *g, const float *angles, const double *sino, double *img, int nS,
+                       int nthreads);
 
 /* Threads the oracle will use when nthreads <= 0. */
 int orc_default_threads(void);

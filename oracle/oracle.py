@@ -41,6 +41,7 @@ class OrcCone(ctypes.Structure):
         ("dShift", ctypes.c_double), ("nX", ctypes.c_int), ("nY", ctypes.c_int), ("nZ", ctypes.c_int),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("xmin", ctypes.c_double),
         ("ymin", ctypes.c_double), ("zmin", ctypes.c_double), ("rFOV", ctypes.c_double),
+        ("short_scan", ctypes.c_int),
     ]
 
 
@@ -87,12 +88,18 @@ def _load():
     lib.orc_cone_taps.restype = None
     lib.orc_cone_preweight.argtypes = [c, d]
     lib.orc_cone_preweight.restype = None
-    lib.orc_cone_filter.argtypes = [c, d, d, i, i]
-    lib.orc_cone_filter.restype = None
-    for name in ("orc_cone_bp", "orc_cone_fdk"):
-        fn = getattr(lib, name)
-        fn.argtypes = [c, f, d, d, i, i]
-        fn.restype = None
+    lib.orc_cone_filter.argtypes = [c, f, d, d, i, i]
+    lib.orc_cone_filter.restype = i
+    lib.orc_cone_bp.argtypes = [c, f, d, d, i, i]
+    lib.orc_cone_bp.restype = None
+    lib.orc_cone_fdk.argtypes = [c, f, d, d, i, i]
+    lib.orc_cone_fdk.restype = i
+    lib.orc_parker_weights.argtypes = [f, i, d, i, ctypes.c_double, d]
+    lib.orc_parker_weights.restype = i
+    lib.orc_parker_fan.argtypes = [g, f, d]
+    lib.orc_parker_fan.restype = i
+    lib.orc_fbp_short_scan.argtypes = [g, f, d, d, i, i]
+    lib.orc_fbp_short_scan.restype = i
     lib.orc_default_threads.argtypes = []
     lib.orc_default_threads.restype = i
     _lib = lib
@@ -210,6 +217,20 @@ class Oracle:
         _load().orc_fbp_adjoint(ctypes.byref(self.g), _fptr(self.angles), _dptr(x), _dptr(out), x.shape[0], nthreads)
         return out
 
+    def parker_weights(self):
+        w = np.zeros(self.sino_shape)
+        if _load().orc_parker_fan(ctypes.byref(self.g), _fptr(self.angles), _dptr(w)) != 0:
+            raise ValueError("short scan: the views do not cover pi + the fan angle")
+        return w
+
+    def fbp_short_scan(self, sino, nthreads=0):
+        p = self._sino(sino)
+        out = np.zeros((p.shape[0],) + self.img_shape, dtype=np.float64)
+        if _load().orc_fbp_short_scan(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0],
+                                      nthreads) != 0:
+            raise ValueError("short scan: the views do not cover pi + the fan angle")
+        return out
+
     def fdk(self, proj, n_rows=1, n_z=1, zmin=None, nthreads=0):
         """proj [nS, nViews, nRows, nCols] (or [nS, nViews, nCols] when n_rows == 1)."""
         p = np.ascontiguousarray(np.asarray(proj, dtype=np.float64))
@@ -227,7 +248,7 @@ class ConeOracle:
     """FP64 cone-beam FDK oracle for one geometry (a dict with the fields of
     ``fbct_cone_geom``; values are taken as float32, as the GPU plan receives them)."""
 
-    _INT = ("nViews", "detector", "nCols", "nRows", "nX", "nY", "nZ")
+    _INT = ("nViews", "detector", "nCols", "nRows", "nX", "nY", "nZ", "short_scan")
 
     def __init__(self, geom: dict, angles):
         _load()
@@ -265,7 +286,9 @@ class ConeOracle:
     def filter(self, proj, nthreads=0):
         p = self._proj(proj)
         out = np.zeros_like(p)
-        _load().orc_cone_filter(ctypes.byref(self.g), _dptr(p), _dptr(out), p.shape[0], nthreads)
+        if _load().orc_cone_filter(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0],
+                                   nthreads) != 0:
+            raise ValueError("short scan: the views do not cover pi + the fan angle")
         return out
 
     def backproject(self, q, nthreads=0):
@@ -277,5 +300,7 @@ class ConeOracle:
     def fdk(self, proj, nthreads=0):
         p = self._proj(proj)
         out = np.zeros((p.shape[0],) + self.vol_shape)
-        _load().orc_cone_fdk(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0], nthreads)
+        if _load().orc_cone_fdk(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0],
+                                nthreads) != 0:
+            raise ValueError("short scan: the views do not cover pi + the fan angle")
         return out

@@ -1,0 +1,51 @@
+"""GPU: short-scan reconstruction with Parker's weights (fan FBP and cone FDK) against
+the FP64 oracle.  Tolerance: relative L2 1e-5 (FP32 arithmetic; the weights are FP32
+copies of the FP64 values)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+from fanbeamctrecon_tensorflowop_amd import workloads
+from fanbeamctrecon_tensorflowop_amd.cone import ConePlan
+from fanbeamctrecon_tensorflowop_amd.plan import FanBeamPlan
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_fan_short_scan_fbp_matches_oracle(oracle_mod):
+    nv_full = 984
+    p, a = workloads.clinical_scan(2)
+    fan = float(p[4]) * float(p[8]) * math.pi / 180
+    nv = int(math.ceil((math.pi + fan) / (2 * math.pi / nv_full))) + 1
+    ps = p.copy()
+    ps[2], ps[3] = nv, nv * 360.0 / nv_full
+    plan = FanBeamPlan(ps, a[:nv])
+    full = FanBeamPlan(p, a)
+    img = workloads.clinical_phantom(2)
+    sino = full.fp(torch.from_numpy(img).cuda())[:, :nv].contiguous()
+    got = plan.fbp_short_scan(sino).double().cpu().numpy()
+    ref = oracle_mod.Oracle(ps, a[:nv]).fbp_short_scan(sino.double().cpu().numpy())
+    e = rel_l2(got, ref)
+    _say("fan short-scan fbp (%d views) rel_l2=%.3e" % (nv, e))
+    assert e <= 1e-5
+
+
+@pytest.mark.parametrize("detector", ["flat", "curved"])
+def test_cone_short_scan_fdk_matches_oracle(oracle_mod, detector):
+    g, a = workloads.cone_scan(nx=96, nz=24, dx=0.8, n_views=240, n_cols=192, n_rows=48, pitch=0.9, R=250.0,
+                               D=480.0, detector=detector, short_scan=True)
+    ells = workloads.random_ellipsoids(6, seed=9, rfov=38.4, zhalf=9.6)
+    P = workloads.ellipsoid_projections(ells, g, a).numpy().astype(np.float32)
+    plan = ConePlan(g, a)
+    got = plan.fdk(torch.from_numpy(P).cuda())[0].double().cpu().numpy()
+    ref = oracle_mod.ConeOracle(g, a).fdk(P[None].astype(np.float64))[0]
+    e = rel_l2(got, ref)
+    _say("cone %s short-scan fdk (%d views) rel_l2=%.3e" % (detector, len(a), e))
+    assert e <= 1e-5

@@ -192,3 +192,26 @@ def test_torch_ops_registered_with_fake_shapes():
     assert torch.ops.fbct.back_projection(meta_sino, al, pl, vol, proj).shape == (2, 16, 16)
     with pytest.raises(ValueError, match="no CPU path"):
         ops.fp_fan_curve(torch.zeros(2, 16, 16), a, p, vol, proj)
+
+
+def test_parker_weights_match_oracle(oracle_mod):
+    """fbct_plan_parker_weights (host) equals the oracle's weights to float32 rounding;
+    a full scan is accepted too (views past 180 deg + fan get weight 0)."""
+    import math
+    nv_full = 360
+    p, a = workloads.scan(nx=64, rfov=32.0, sid=200.0, sdd=400.0, n_cols=96, dcol_rad=0.4 / 96, n_views=nv_full,
+                          dshift=0.75)
+    nv = int(math.ceil((math.pi + 0.4 + 0.75 * 2 * 0.4 / 96) / (2 * math.pi / nv_full))) + 2
+    ps = p.copy()
+    ps[2], ps[3] = nv, nv * 360 / nv_full
+    plan = FanBeamPlan(ps, a[:nv])
+    w = plan.parker_weights()
+    ref = oracle_mod.Oracle(ps, a[:nv]).parker_weights()
+    np.testing.assert_allclose(w, ref, rtol=0, atol=3e-7)
+    assert w.max() == np.float32(2.0) and w.min() == 0.0
+    ps2, _ = workloads.scan(nx=64, rfov=32.0, sid=200.0, sdd=400.0, n_cols=96, dcol_rad=0.4 / 96, n_views=170,
+                            angular_range=170.0)
+    short = FanBeamPlan(ps2, (np.arange(170) * (math.pi / 180)).astype(np.float32))  # 170 deg of views
+    with pytest.raises(_lib.FbctError) as e:
+        short.parker_weights()
+    assert e.value.status == 2
