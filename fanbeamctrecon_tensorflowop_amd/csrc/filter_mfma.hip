@@ -12,10 +12,10 @@
 // [Np][Kp]); P is split in registers as it is loaded (optionally multiplied by a
 // per-sample weight first, the cone-beam pre-weight; a BF16 input needs no split).
 //
-// Tiling: a wave computes 16 rows x 64 columns (4 MFMA tiles) over K in steps of 32;
-// a workgroup of 4 waves covers 64 rows.  Operand fragments are loaded straight from
-// global memory (P rows as FP32, H rows as BF16: H is symmetric, so the B fragment of
-// column c is row c of H, contiguous in k).
+// Tiling: a workgroup of 4 waves computes 128 rows x 128 columns; per K step of 32 it
+// stages the BF16 planes of its P rows (split once per element) and of its H columns
+// (H is symmetric, so the B fragment of column c is row c of H, contiguous in k) in LDS,
+// and each wave multiplies its 64 x 64 quadrant (4 x 4 MFMA tiles).
 // Lane layout of v_mfma_f32_16x16x32_bf16: A lane l holds row l%16, k = 8*(l/16)+j;
 // B lane l holds column l%16, k = 8*(l/16)+j; D lane l holds rows 4*(l/16)+j, column l%16.
 
@@ -29,8 +29,6 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kMfmaK = 32;
-constexpr int kWaveCols = 64;  // 4 tiles of 16
 
 struct MfmaArgs {
     const void *in;    // TI[rows][nC]
@@ -54,127 +52,171 @@ __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &
 __device__ __forceinline__ float widen(float x) { return x; }
 __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// kAligned: every row starts 16-byte aligned, so the 8 k-values of an A fragment load
-// as 16 bytes.  TI: input storage (float or BF16 bits).  kW: inputs are multiplied by a
-// per-sample weight before the split.  A BF16 input without weights is exactly its own
-// hi part, so only the three products with the H planes are formed.
-// kPairs: the output is written as column pairs, out2[r][c] = (q[r][c], q[r][min(c+1, nC-1)])
-// (float2), the layout the cone backprojection reads its two column taps from.
-template <bool kAligned, typename TI, bool kW, bool kPairs>
-__global__ __launch_bounds__(256) void ramp_filter_mfma_kernel(MfmaArgs A)
+// LDS-tiled version.  A workgroup (4 waves) computes 128 rows x 128 columns; per K step
+// of 32 it stages the three BF16 planes of its P rows (split once per element, weights
+// applied first) and of its H columns in LDS, then each wave multiplies its 64 x 64
+// quadrant (4 x 4 MFMA tiles) from LDS.  Rows of the LDS planes are padded to 40 BF16
+// so the 16 rows a fragment read touches fall in distinct banks.
+constexpr int kBM = 128, kBN = 128, kBK = 32, kKS = 40;
+
+template <typename TI, bool kW, bool kPairs>
+__global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 {
     constexpr bool kExactA = sizeof(TI) == 2 && !kW;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const long long r0 = (long long)blockIdx.x * 64 + wave * 16;
-    const int c0 = blockIdx.y * kWaveCols;
-    const long long arow = r0 + (lane & 15);
+    constexpr int kPA = kExactA ? 1 : 3;  // A planes in use
+    __shared__ __bf16 As[kPA][kBM][kKS];
+    __shared__ __bf16 Bs[3][kBN][kKS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const long long r0 = (long long)blockIdx.x * kBM;
+    const int c0 = blockIdx.y * kBN;
+    // staging assignment: thread -> (row or column tid / 2, 16 k-values at kh)
+    const int srow = tid >> 1, kh = (tid & 1) * 16;
+    const long long arow = r0 + srow;
     const bool arow_ok = arow < A.rows;
     const TI *prow = static_cast<const TI *>(A.in) + (arow_ok ? arow : 0) * (long long)A.nC;
     const float *wrow = kW && A.w ? A.w + (arow_ok ? arow % A.wrows : 0) * (long long)A.nC : nullptr;
-    const float *w2row = kW && A.w2 ? A.w2 + (arow_ok ? (arow / A.w2div) % A.w2rows : 0) * (long long)A.nC : nullptr;
-    const int kl = 8 * (lane >> 4);  // this lane's k offset within a 32-wide step
+    const float *w2row =
+        kW && A.w2 ? A.w2 + (arow_ok ? (arow / A.w2div) % A.w2rows : 0) * (long long)A.nC : nullptr;
+    const bool vec = ((size_t)A.nC * sizeof(TI)) % 16 == 0 && reinterpret_cast<uintptr_t>(A.in) % 16 == 0;
     const size_t plane = (size_t)A.Np * A.Kp;
+    const __bf16 *hcol = A.hp + (size_t)(c0 + srow) * A.Kp;  // < Np: planes are zero-padded
 
-    f32x4 acc[4];
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[i][t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (int k0 = 0; k0 < A.Kp; k0 += kMfmaK) {
-        // A fragment: 8 consecutive k of one row, split into three BF16 planes
-        float x[8];
-        const int kb = k0 + kl;
-        const bool full = arow_ok && kb + 8 <= A.nC;
-        if (kAligned && full) {
-            if constexpr (sizeof(TI) == 4) {
-                const float4 u = *reinterpret_cast<const float4 *>(prow + kb);
-                const float4 v = *reinterpret_cast<const float4 *>(prow + kb + 4);
-                x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
+    for (int k0 = 0; k0 < A.Kp; k0 += kBK) {
+        // ---- stage A: 16 consecutive k of one row, weighted and split
+        {
+            float x[16];
+            const int kb = k0 + kh;
+            if (vec && arow_ok && kb + 16 <= A.nC) {
+                if constexpr (sizeof(TI) == 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float4 u = *reinterpret_cast<const float4 *>(prow + kb + 4 * q);
+                        x[4 * q] = u.x, x[4 * q + 1] = u.y, x[4 * q + 2] = u.z, x[4 * q + 3] = u.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const uint4 u = *reinterpret_cast<const uint4 *>(prow + kb + 8 * q);
+                        const uint32_t d[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            x[8 * q + 2 * j] = __uint_as_float(d[j] << 16);
+                            x[8 * q + 2 * j + 1] = __uint_as_float(d[j] & 0xffff0000u);
+                        }
+                    }
+                }
             } else {
-                const uint4 u = *reinterpret_cast<const uint4 *>(prow + kb);
-                const uint32_t q[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    x[2 * j] = __uint_as_float(q[j] << 16);
-                    x[2 * j + 1] = __uint_as_float(q[j] & 0xffff0000u);
+                for (int j = 0; j < 16; ++j) x[j] = (arow_ok && kb + j < A.nC) ? widen(prow[kb + j]) : 0.0f;
+            }
+            if constexpr (kW) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const bool ok = arow_ok && kb + j < A.nC;
+                    float wt = ok ? 1.0f : 0.0f;
+                    if (wrow) wt *= ok ? wrow[kb + j] : 0.0f;
+                    if (w2row) wt *= ok ? w2row[kb + j] : 0.0f;
+                    x[j] *= wt;
                 }
             }
-        } else {
+            bf16x8 h[2], l[2], m[2];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = (arow_ok && kb + j < A.nC) ? widen(prow[kb + j]) : 0.0f;
-        }
-        if constexpr (kW) {
+            for (int j = 0; j < 16; ++j) {
+                if constexpr (kExactA) {
+                    h[j >> 3][j & 7] = (__bf16)x[j];
+                } else {
+                    __bf16 a, b, c;
+                    split3(x[j], a, b, c);
+                    h[j >> 3][j & 7] = a, l[j >> 3][j & 7] = b, m[j >> 3][j & 7] = c;
+                }
+            }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const bool ok = arow_ok && kb + j < A.nC;
-                float wt = ok ? 1.0f : 0.0f;
-                if (wrow) wt *= ok ? wrow[kb + j] : 0.0f;
-                if (w2row) wt *= ok ? w2row[kb + j] : 0.0f;
-                x[j] *= wt;
+            for (int q = 0; q < 2; ++q) {
+                *reinterpret_cast<bf16x8 *>(&As[0][srow][kh + 8 * q]) = h[q];
+                if constexpr (!kExactA) {
+                    *reinterpret_cast<bf16x8 *>(&As[1][srow][kh + 8 * q]) = l[q];
+                    *reinterpret_cast<bf16x8 *>(&As[2][srow][kh + 8 * q]) = m[q];
+                }
             }
         }
-        bf16x8 ah, al, all;
+        // ---- stage B: 16 consecutive k of one column of H, three planes
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if constexpr (kExactA) {
-                ah[j] = (__bf16)x[j];
-            } else {
-                __bf16 h, l, m;
-                split3(x[j], h, l, m);
-                ah[j] = h, al[j] = l, all[j] = m;
+        for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                *reinterpret_cast<bf16x8 *>(&Bs[pl][srow][kh + 8 * q]) =
+                    *reinterpret_cast<const bf16x8 *>(hcol + pl * plane + k0 + kh + 8 * q);
+        __syncthreads();
+
+        // ---- multiply this wave's 64 x 64 quadrant
+        const int fr = lane & 15, fk = 8 * (lane >> 4);
+        bf16x8 ah[4], al[4], all[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = wr * 64 + 16 * i + fr;
+            ah[i] = *reinterpret_cast<const bf16x8 *>(&As[0][r][fk]);
+            if constexpr (!kExactA) {
+                al[i] = *reinterpret_cast<const bf16x8 *>(&As[1][r][fk]);
+                all[i] = *reinterpret_cast<const bf16x8 *>(&As[2][r][fk]);
             }
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int col = c0 + 16 * t + (lane & 15);  // < Np: planes are zero-padded
-            const size_t off = (size_t)col * A.Kp + kb;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(A.hp + off);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(A.hp + plane + off);
-            const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(A.hp + 2 * plane + off);
-            if constexpr (!kExactA) {
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(all, bh, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bl, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+            const int c = wc * 64 + 16 * t + fr;
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bs[0][c][fk]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bs[1][c][fk]);
+            const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(&Bs[2][c][fk]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if constexpr (!kExactA) {
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(all[i], bh, acc[i][t], 0, 0, 0);
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl, acc[i][t], 0, 0, 0);
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][t], 0, 0, 0);
+                }
+                acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bll, acc[i][t], 0, 0, 0);
+                acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][t], 0, 0, 0);
+                acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][t], 0, 0, 0);
             }
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bll, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
         }
+        __syncthreads();  // the next K step overwrites the planes
     }
+
+    // ---- store: D lane l holds rows 4 (l / 16) + j, column l % 16 of each 16 x 16 tile
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int col = c0 + 16 * t + (lane & 15);
+        const int col = c0 + wc * 64 + 16 * t + (lane & 15);
         if (col >= A.nC) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long long row = r0 + 4 * (lane >> 4) + j;
-            if (row >= A.rows) continue;
-            if constexpr (kPairs) {
-                float *o = A.out + 2 * (row * A.nC + col);
-                o[0] = acc[t][j];
-                if (col > 0) o[-1] = acc[t][j];
-                if (col == A.nC - 1) o[1] = acc[t][j];
-            } else {
-                A.out[row * A.nC + col] = acc[t][j];
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long row = r0 + wr * 64 + 16 * i + 4 * (lane >> 4) + j;
+                if (row >= A.rows) continue;
+                if constexpr (kPairs) {
+                    float *o = A.out + 2 * (row * A.nC + col);
+                    o[0] = acc[i][t][j];
+                    if (col > 0) o[-1] = acc[i][t][j];
+                    if (col == A.nC - 1) o[1] = acc[i][t][j];
+                } else {
+                    A.out[row * A.nC + col] = acc[i][t][j];
+                }
             }
-        }
     }
-}
-
-template <typename TI, bool kW, bool kPairs>
-void launch_mfma_t(bool aligned, dim3 grid, hipStream_t stream, const MfmaArgs &A)
-{
-    if (aligned)
-        hipLaunchKernelGGL((ramp_filter_mfma_kernel<true, TI, kW, kPairs>), grid, dim3(256), 0, stream, A);
-    else
-        hipLaunchKernelGGL((ramp_filter_mfma_kernel<false, TI, kW, kPairs>), grid, dim3(256), 0, stream, A);
 }
 
 template <typename TI, bool kW>
-void launch_mfma_p(bool pairs, bool aligned, dim3 grid, hipStream_t stream, const MfmaArgs &A)
+void launch_gemm_p(bool pairs, dim3 grid, hipStream_t stream, const MfmaArgs &A)
 {
-    if (pairs) launch_mfma_t<TI, kW, true>(aligned, grid, stream, A);
-    else launch_mfma_t<TI, kW, false>(aligned, grid, stream, A);
+    if (pairs) hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, true>), grid, dim3(256), 0, stream, A);
+    else hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, false>), grid, dim3(256), 0, stream, A);
 }
 
 }  // namespace
@@ -199,20 +241,20 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     A.wrows = rw.w ? rw.wrows : 1;
     A.w2div = rw.w2 ? rw.w2div : 1;
     A.w2rows = rw.w2 ? rw.w2rows : 1;
-    const long long gx = (rows + 63) / 64;
+    if (H.Np % kBN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC)
+        return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: operand planes not padded to the tile");
+    const long long gx = (rows + kBM - 1) / kBM;
     if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
-    dim3 grid((unsigned)gx, (H.nC + kWaveCols - 1) / kWaveCols);
-    const size_t esz = ti == FBCT_BF16 ? 2 : 4;
-    const bool aligned = reinterpret_cast<uintptr_t>(in) % 16 == 0 && ((size_t)H.nC * esz) % 16 == 0;
+    dim3 grid((unsigned)gx, (H.nC + kBN - 1) / kBN);
     if (ti == FBCT_BF16) {
-        if (w) launch_mfma_p<uint16_t, true>(pairs, aligned, grid, stream, A);
-        else launch_mfma_p<uint16_t, false>(pairs, aligned, grid, stream, A);
+        if (w) launch_gemm_p<uint16_t, true>(pairs, grid, stream, A);
+        else launch_gemm_p<uint16_t, false>(pairs, grid, stream, A);
     } else {
-        if (w) launch_mfma_p<float, true>(pairs, aligned, grid, stream, A);
-        else launch_mfma_p<float, false>(pairs, aligned, grid, stream, A);
+        if (w) launch_gemm_p<float, true>(pairs, grid, stream, A);
+        else launch_gemm_p<float, false>(pairs, grid, stream, A);
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_filter_mfma_kernel launch: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_gemm_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

@@ -385,7 +385,7 @@ static void build_tables(fbct_plan &p)
     // MFMA filter operand: H[m][c] = h[c - m] (symmetric), split into three BF16 planes
     // hi + lo + ll (each the BF16 rounding of what the previous planes left), padded with
     // zeros to Np = 64-multiple rows and Kp = 32-multiple columns.
-    d.ramp_np = (nC + 63) / 64 * 64;
+    d.ramp_np = (nC + 127) / 128 * 128;
     d.ramp_kp = (nC + 31) / 32 * 32;
     toeplitz_planes(p.ramp, nC, d.ramp_np, d.ramp_kp, p.ramp_planes);
 
