@@ -210,7 +210,7 @@ fbct_status fbct_fbp_short_scan(fbct_plan *plan, const float *sino, float *image
     fbct::RowWeights rw;
     rw.w = pk;  // row r = slice * nViews + view
     rw.wrows = plan->g.nViews;
-    if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, (int64_t)n_slices * plan->g.nViews, rw, st, false)) !=
+    if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, (int64_t)n_slices * plan->g.nViews, rw, st)) !=
         FBCT_OK)
         return s;
     return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);

@@ -220,7 +220,7 @@ fbct_status check_dtype(int d, const char *what)
 }
 
 fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void *proj, fbct_dtype ti, float *q, int n,
-                       bool pairs, hipStream_t st)
+                       hipStream_t st)
 {
     RampOperand H{t.ramp_planes, p->g.nCols, p->d.ramp_np, p->d.ramp_kp};
     const int64_t rows = (int64_t)n * p->g.nViews * p->g.nRows;
@@ -230,7 +230,7 @@ fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void 
     rw.w2 = t.parker;    // [nViews][nCols] Parker weights (short scan)
     rw.w2div = p->g.nRows;
     rw.w2rows = p->g.nViews;
-    return launch_ramp_gemm(H, proj, ti, q, rows, rw, st, pairs);
+    return launch_ramp_gemm(H, proj, ti, q, rows, rw, st);
 }
 
 }  // namespace
@@ -316,7 +316,7 @@ fbct_status fbct_cone_plan_destroy(fbct_cone_plan *plan)
 size_t fbct_cone_workspace_size(const fbct_cone_plan *plan, int n)
 {
     if (!plan || n <= 0) return 0;
-    return fbct::proj_elems(plan, n) * 2 * sizeof(float);
+    return fbct::proj_elems(plan, n) * sizeof(float);
 }
 
 fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dtype, float *filtered, int n,
@@ -327,7 +327,7 @@ fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dt
     if ((s = fbct::check_dtype(proj_dtype, "fbct_cone_filter")) != FBCT_OK) return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), filtered, n, false,
+    return fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), filtered, n,
                             static_cast<hipStream_t>(stream));
 }
 
@@ -339,7 +339,7 @@ fbct_status fbct_cone_backproject(fbct_cone_plan *plan, const float *filtered, v
     if ((s = fbct::check_dtype(vol_dtype, "fbct_cone_backproject")) != FBCT_OK) return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
-    return fbct::launch_cone_bp(*plan, *t, filtered, false, vol, static_cast<fbct_dtype>(vol_dtype), n,
+    return fbct::launch_cone_bp(*plan, *t, filtered, vol, static_cast<fbct_dtype>(vol_dtype), n,
                                 static_cast<hipStream_t>(stream));
 }
 
@@ -359,11 +359,9 @@ fbct_status fbct_cone_fdk(fbct_cone_plan *plan, const void *proj, int proj_dtype
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float *q = static_cast<float *>(workspace);
-    // the filter writes column pairs, so the backprojection reads both column taps of a
-    // row with one 8-byte load
-    if ((s = fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), q, n, true, st)) != FBCT_OK)
+    if ((s = fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), q, n, st)) != FBCT_OK)
         return s;
-    return fbct::launch_cone_bp(*plan, *t, q, true, vol, static_cast<fbct_dtype>(vol_dtype), n, st);
+    return fbct::launch_cone_bp(*plan, *t, q, vol, static_cast<fbct_dtype>(vol_dtype), n, st);
 }
 
 }  // extern "C"

@@ -8,9 +8,14 @@
 // Here one launch covers all views and volumes.  A thread owns an (x, y) column of KZ
 // voxels of one volume, accumulates all views in registers and stores once.  Per
 // (column, view) the column coordinate, its taps and weight and the in-range decision
-// are computed once; per voxel only the row coordinate (one fma), the two row taps
-// and a bilinear blend.  Texture semantics are reproduced with exact FP32 weights.
-// Near the two fan edges the in-range decision is retaken in FP64 on t/U.
+// are computed once; per voxel only the row coordinate (one fma), the taps and a
+// bilinear blend.  Texture semantics are reproduced with exact FP32 weights.  Near the
+// two fan edges the in-range decision is retaken in FP64 on t/U.
+//
+// Volumes of 16 slices or more use cone_bp_lds_kernel, which stages each workgroup's
+// detector footprint in LDS (the taps of a 16 x 16 x 16 voxel tile span about 30 x 25
+// detector samples per view, read by 4096 voxels); thinner volumes read the taps from
+// global memory (cone_bp_kernel).
 
 #include <hip/hip_runtime.h>
 
@@ -24,7 +29,7 @@ constexpr int kTx = 16, kTy = 16;
 constexpr float kEdgeBand = 0.01f;  // columns
 
 struct ConeArgs {
-    const float *q;        // [n][nViews][nRows][nCols] filtered projections (float2 pairs: see kPairs)
+    const float *q;        // [n][nViews][nRows][nCols] filtered projections
     void *vol;             // TO[n][nZ][nY][nX]
     const float2 *cs;
     const double2 *csd;
@@ -51,9 +56,7 @@ __device__ __forceinline__ float fan_atan(const ConeArgs &A, float z)
 
 // kFlat: flat detector (xi linear in t/U, weight 1/U^2); else curved (atan, 1/(U^2+t^2)).
 // kNc: atan polynomial terms (curved only).  KZ: voxels per thread along z.
-// kPairs: q holds column pairs (q[c], q[min(c+1, nC-1)]) as float2, so the two column
-// taps of a row are one 8-byte load.
-template <bool kFlat, int kNc, int KZ, bool kPairs, typename TO>
+template <bool kFlat, int kNc, int KZ, typename TO>
 __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
 {
     unsigned t, gz;  // XCD-aware (tile, z-chunk of a volume): an XCD keeps whole z-chunks
@@ -79,9 +82,8 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
         const float cmax = (float)(nC - 1), rmax = (float)(nR - 1);
         const float cfl = (float)max(nC - 2, 0), rfl = (float)max(nR - 2, 0);
         const size_t view_elems = (size_t)nR * nC;
-        constexpr int kStride = kPairs ? 2 : 1;  // floats per detector sample
-        const float *qv = A.q + vi * (size_t)A.nViews * view_elems * kStride;
-        for (int v = 0; v < A.nViews; ++v, qv += view_elems * kStride) {
+        const float *qv = A.q + vi * (size_t)A.nViews * view_elems;
+        for (int v = 0; v < A.nViews; ++v, qv += view_elems) {
             const float2 cs = A.cs[v];
             const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
             const float t = fmaf(x, cs.y, -y * cs.x);
@@ -109,17 +111,10 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
             const float rt_a = fmaf(zk[0], a, A.zr_offset), rt_b = fmaf(zk[KZ - 1], a, A.zr_offset);
             const bool inside = fminf(rt_a, rt_b) >= 0.0f && fmaxf(rt_a, rt_b) < rmax;
             auto tap_rows = [&](int r0, int r1, float &top, float &bot) {
-                if constexpr (kPairs) {
-                    const float2 u = *reinterpret_cast<const float2 *>(qv + 2 * (r0 * nC + c0));
-                    const float2 d = *reinterpret_cast<const float2 *>(qv + 2 * (r1 * nC + c0));
-                    top = fmaf(al, u.y - u.x, u.x);
-                    bot = fmaf(al, d.y - d.x, d.x);
-                } else {
-                    const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
-                    const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
-                    top = fmaf(al, p01 - p00, p00);
-                    bot = fmaf(al, p11 - p10, p10);
-                }
+                const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
+                const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
+                top = fmaf(al, p01 - p00, p00);
+                bot = fmaf(al, p11 - p10, p10);
             };
             if (inside) {
 #pragma unroll
@@ -152,35 +147,242 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
         if (k0 + k < A.nZ) store_f(out, (size_t)(k0 + k) * A.nY * A.nX, acc[k]);
 }
 
-template <bool kFlat, int kNc, int KZ, bool kPairs>
+// LDS-staged variant for z-chunks of kLdsKZ voxels (plain layout).  The workgroup's
+// detector footprint per view is derived from the four corner columns of its 16 x 16
+// tile (t/U, and so the column coordinate, is extremal at a corner; the row coordinate at
+// a corner and an end of the z-chunk), kLdsGroup views at a time: lane 4m + c evaluates
+// corner c of view m.  Per batch of VB views the workgroup stages each footprint's rows
+// once (one detector row per wave instruction, lane = column, indices clamped to the
+// detector, which reproduces clamp-to-edge), and every voxel reads its four taps from
+// LDS.  A column whose taps leave the staged window (a footprint taller than kLdsRows or
+// wider than kLdsCols) reads them from global memory instead.
+constexpr int kLdsKZ = 16;
+constexpr int kLdsVB = 2;                      // views per staging batch (2: 23 KB of LDS)
+constexpr int kLdsGroup = 16;                  // views per footprint evaluation (64 lanes / 4 corners)
+constexpr int kLdsCols = 64;                   // staged columns per row: one per lane
+constexpr int kLdsPitch = kLdsCols + 1;        // floats between the views of a staged row
+constexpr int kLdsRows = 44;                   // staged rows per view
+// Staged element (view m, row r, column c) sits at r * stride + m * kLdsPitch + c: the
+// four taps of a voxel of any view of the batch are two ds_read2 with immediate offsets
+// (at most 255 dwords), and the odd row stride keeps adjacent rows off the same banks.
+template <int VB>
+constexpr int lds_stride()
+{
+    return (VB * kLdsPitch) | 1;
+}
+
+__device__ __forceinline__ int floor_to_int(float v)
+{
+    return (int)floorf(fminf(fmaxf(v, -1.0e6f), 1.0e6f));
+}
+
+// floor(v) as an integer in one instruction (v_cvt_flr_i32_f32); v is a finite row
+// coordinate well inside the int range.
+__device__ __forceinline__ int cvt_floor(float v)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+template <bool kFlat, int kNc, int VB, typename TO>
+__global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
+{
+    constexpr int KZ = kLdsKZ;
+    constexpr int RS = lds_stride<VB>();
+    static_assert(kLdsGroup % VB == 0, "a batch of views lies inside one footprint group");
+    static_assert(RS + (VB - 1) * kLdsPitch + 1 <= 255, "tap offsets fit ds_read2");
+    __shared__ float stage[kLdsRows * RS];
+    unsigned t, gz;
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, gz);
+    const int tx0 = (int)(t % gridDim.x) * kTx, ty0 = (int)(t / gridDim.x) * kTy;
+    const int i = tx0 + (int)threadIdx.x % kTx;
+    const int j = ty0 + (int)threadIdx.x / kTx;
+    const int zc = gz % A.nzc;
+    const size_t vi = gz / A.nzc;
+    const int k0 = zc * KZ;
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const float x = fmaf(A.dx, (float)i + 0.5f, A.xmin);
+    const float y = fmaf(A.dy, (float)j + 0.5f, A.ymin);
+    // no early exit: every thread takes part in the staging barriers
+    const bool active = i < A.nX && j < A.nY && x * x + y * y <= A.r2;
+    // Row coordinate of voxel k: rt_k = rt_0 + k (dz a), rt_0 = z_0 a + zr_offset and
+    // a = zr_scale / U -- one fma with k as an immediate.
+    const float z0 = fmaf(A.dz, (float)k0 + 0.5f, A.zmin);
+    const float z1 = fmaf(A.dz, (float)(k0 + KZ - 1) + 0.5f, A.zmin);
+    float acc[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) acc[k] = 0.0f;
+    const int ci = min(tx0 + ((lane & 1) ? kTx - 1 : 0), A.nX - 1);
+    const int cj = min(ty0 + ((lane & 2) ? kTy - 1 : 0), A.nY - 1);
+    const float cx = fmaf(A.dx, (float)ci + 0.5f, A.xmin);
+    const float cy = fmaf(A.dy, (float)cj + 0.5f, A.ymin);
+    const double xd = (double)A.xmin + (double)A.dx * ((double)i + 0.5);
+    const double yd = (double)A.ymin + (double)A.dy * ((double)j + 0.5);
+    const int nC = A.nCols, nR = A.nRows;
+    const float cmax = (float)(nC - 1), rmax = (float)(nR - 1);
+    const float cfl = (float)max(nC - 2, 0), rfl = (float)max(nR - 2, 0);
+    const size_t view_elems = (size_t)nR * nC;
+    const float *qvol = A.q + vi * (size_t)A.nViews * view_elems;
+
+    for (int vg = 0; vg < A.nViews; vg += kLdsGroup) {
+        // footprints of views vg + m, m < 16: after the 4-lane reduction lane 4m holds
+        // the first staged column, the first staged row and the last staged row (a margin
+        // of one sample on each side absorbs rounding between corner and voxel)
+        int f_c, f_r0, f_r1;
+        {
+            const float2 cs = A.cs[min(vg + (lane >> 2), A.nViews - 1)];
+            const float U = fmaf(-cy, cs.y, fmaf(-cx, cs.x, A.R));
+            const float tt = fmaf(cx, cs.y, -cy * cs.x);
+            const float rU = __builtin_amdgcn_rcpf(U);
+            const float zz = tt * rU;
+            float ctn = fmaf(kFlat ? zz : fan_atan<kNc>(A, zz), A.xi_scale, A.xi_offset) - 0.5f;
+            const float aa = rU * A.zr_scale;
+            const float ra = fmaf(z0, aa, A.zr_offset), rb = fmaf(z1, aa, A.zr_offset);
+            float rtn = fminf(ra, rb), rtx = fmaxf(ra, rb);
+#pragma unroll
+            for (int sh = 1; sh <= 2; sh <<= 1) {
+                ctn = fminf(ctn, __shfl_xor(ctn, sh));
+                rtn = fminf(rtn, __shfl_xor(rtn, sh));
+                rtx = fmaxf(rtx, __shfl_xor(rtx, sh));
+            }
+            f_c = floor_to_int(ctn) - 1;
+            f_r0 = floor_to_int(rtn) - 1;
+            f_r1 = floor_to_int(rtx) + 2;
+        }
+        const int vend = min(vg + kLdsGroup, A.nViews);
+        for (int v0 = vg; v0 < vend; v0 += VB) {
+            int c_lo[VB], r_lo[VB], nrow[VB];
+#pragma unroll
+            for (int m = 0; m < VB; ++m) {
+                const int src = 4 * (v0 - vg + m);
+                c_lo[m] = __builtin_amdgcn_readlane(f_c, src);
+                r_lo[m] = __builtin_amdgcn_readlane(f_r0, src);
+                const int r_hi = __builtin_amdgcn_readlane(f_r1, src);
+                nrow[m] = (v0 + m < vend) ? min(max(r_hi - r_lo[m] + 1, 1), kLdsRows) : 0;
+            }
+            __syncthreads();  // the previous batch's reads are done
+#pragma unroll
+            for (int m = 0; m < VB; ++m) {
+                if (nrow[m] == 0) continue;
+                const float *qv = qvol + (size_t)(v0 + m) * view_elems;
+                const unsigned c = (unsigned)min(max(c_lo[m] + lane, 0), nC - 1);
+                float *dst = stage + m * kLdsPitch + lane;
+                for (int i0 = wave; i0 < nrow[m]; i0 += 16) {
+                    float val[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = min(max(r_lo[m] + min(i0 + 4 * u, nrow[m] - 1), 0), nR - 1);
+                        val[u] = (qv + (size_t)r * nC)[c];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) dst[min(i0 + 4 * u, nrow[m] - 1) * RS] = val[u];
+                }
+            }
+            __syncthreads();
+            if (!active) continue;
+#pragma unroll
+            for (int m = 0; m < VB; ++m) {
+                const int v = v0 + m;
+                if (v >= vend) break;
+                const float2 cs = A.cs[v];
+                const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
+                const float tv = fmaf(x, cs.y, -y * cs.x);
+                const float rU = __builtin_amdgcn_rcpf(U);
+                const float z = tv * rU;
+                const float xi = fmaf(kFlat ? z : fan_atan<kNc>(A, z), A.xi_scale, A.xi_offset);
+                bool in = xi > 0.0f && xi < (float)nC;
+                if (fabsf(xi) < kEdgeBand || fabsf(xi - (float)nC) < kEdgeBand) {
+                    const double2 c = A.csd[v];
+                    const double Ud = (double)A.R - xd * c.x - yd * c.y;
+                    const double td = xd * c.y - yd * c.x;
+                    in = (td - Ud * A.tan_min > 0.0) && (td - Ud * A.tan_max < 0.0);
+                }
+                if (!in) continue;
+                const float w = kFlat ? rU * rU : __builtin_amdgcn_rcpf(fmaf(U, U, tv * tv));
+                const float a = rU * A.zr_scale;
+                const float step = a * A.dz;
+                const float rt0 = fmaf(z0, a, A.zr_offset);
+                const float rt1 = fmaf(step, (float)(KZ - 1), rt0);
+                const float ctu = xi - 0.5f;
+                const int cl = (int)floorf(ctu) - c_lo[m];
+                const int rla = (int)floorf(fminf(rt0, rt1)) - r_lo[m];
+                const int rlb = (int)floorf(fmaxf(rt0, rt1)) - r_lo[m];
+                if ((unsigned)cl <= (unsigned)(kLdsCols - 2) && rla >= 0 && rlb <= nrow[m] - 2) {
+                    // unclamped coordinates: the staged rows and columns repeat the edges.
+                    // Rows are blended first, then the columns with weights w (1 - al), w al.
+                    const float al = __builtin_amdgcn_fractf(ctu);
+                    const float wr = w * al, wl = w - wr;
+                    const int base = m * kLdsPitch + cl - r_lo[m] * RS;
+#pragma unroll
+                    for (int k = 0; k < KZ; ++k) {
+                        const float rt = fmaf(step, (float)k, rt0);
+                        const float be = __builtin_amdgcn_fractf(rt);
+                        const float *p = stage + (cvt_floor(rt) * RS + base);
+                        const float left = fmaf(be, p[RS] - p[0], p[0]);
+                        const float right = fmaf(be, p[RS + 1] - p[1], p[1]);
+                        acc[k] = fmaf(wl, left, fmaf(wr, right, acc[k]));
+                    }
+                } else {
+                    const float *qv = qvol + (size_t)v * view_elems;
+                    const float ct = fminf(fmaxf(ctu, 0.0f), cmax);
+                    const float fc = fminf(floorf(ct), cfl);
+                    const float al = ct - fc;
+                    const int c0 = (int)fc, c1 = min(c0 + 1, nC - 1);
+#pragma unroll
+                    for (int k = 0; k < KZ; ++k) {
+                        const float rt = fminf(fmaxf(fmaf(step, (float)k, rt0), 0.0f), rmax);
+                        const float fr = fminf(floorf(rt), rfl);
+                        const float be = rt - fr;
+                        const int r0 = (int)fr, r1 = min(r0 + 1, nR - 1);
+                        const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
+                        const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
+                        const float top = fmaf(al, p01 - p00, p00);
+                        const float bot = fmaf(al, p11 - p10, p10);
+                        acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
+                    }
+                }
+            }
+        }
+    }
+    if (!(i < A.nX && j < A.nY)) return;
+    TO *out = static_cast<TO *>(A.vol) + vi * (size_t)A.nZ * A.nY * A.nX + (size_t)j * A.nX + i;
+#pragma unroll
+    for (int k = 0; k < KZ; ++k)
+        if (k0 + k < A.nZ) store_f(out, (size_t)(k0 + k) * A.nY * A.nX, acc[k]);
+}
+
+template <bool kFlat, int kNc>
+void launch_lds(dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
+{
+    constexpr int VB = kLdsVB;
+    if (to == FBCT_BF16)
+        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, VB, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
+    else
+        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, VB, float>), grid, dim3(kTx * kTy), 0, st, A);
+}
+
+template <bool kFlat, int kNc, int KZ>
 void launch_kz(dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
     if (to == FBCT_BF16)
-        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, kPairs, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
     else
-        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, kPairs, float>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_kernel<kFlat, kNc, KZ, float>), grid, dim3(kTx * kTy), 0, st, A);
 }
 
-template <bool kFlat, int kNc, bool kPairs>
+template <bool kFlat, int kNc>
 void launch_nc(int kz, dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
-    if (kz == 16) launch_kz<kFlat, kNc, 16, kPairs>(grid, st, A, to);
-    else if (kz == 4) launch_kz<kFlat, kNc, 4, kPairs>(grid, st, A, to);
-    else launch_kz<kFlat, kNc, 1, kPairs>(grid, st, A, to);
-}
-
-template <bool kPairs>
-void launch_det(const fbct_cone_plan &p, int kz, dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
-{
-    if (p.g.detector == FBCT_DETECTOR_FLAT) launch_nc<true, 0, kPairs>(kz, grid, st, A, to);
-    else if (p.d.atan_nc == 9) launch_nc<false, 9, kPairs>(kz, grid, st, A, to);
-    else if (p.d.atan_nc == 5) launch_nc<false, 5, kPairs>(kz, grid, st, A, to);
-    else launch_nc<false, 0, kPairs>(kz, grid, st, A, to);
+    if (kz == kLdsKZ) launch_lds<kFlat, kNc>(grid, st, A, to);
+    else if (kz == 4) launch_kz<kFlat, kNc, 4>(grid, st, A, to);
+    else launch_kz<kFlat, kNc, 1>(grid, st, A, to);
 }
 
 }  // namespace
 
-fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, bool pairs,
+fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q,
                            void *vol, fbct_dtype to, int n, hipStream_t stream)
 {
     const fbct_cone_geom &g = p.g;
@@ -215,8 +417,10 @@ fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, c
     const long long gz = (long long)A.nzc * n;
     if (gz > 65535) return fail(FBCT_ERR_UNSUPPORTED, "cone backprojection: %lld z-groups exceed the grid", gz);
     const dim3 grid((g.nX + kTx - 1) / kTx, (g.nY + kTy - 1) / kTy, (unsigned)gz);
-    if (pairs) launch_det<true>(p, kz, grid, stream, A, to);
-    else launch_det<false>(p, kz, grid, stream, A, to);
+    if (g.detector == FBCT_DETECTOR_FLAT) launch_nc<true, 0>(kz, grid, stream, A, to);
+    else if (p.d.atan_nc == 9) launch_nc<false, 9>(kz, grid, stream, A, to);
+    else if (p.d.atan_nc == 5) launch_nc<false, 5>(kz, grid, stream, A, to);
+    else launch_nc<false, 0>(kz, grid, stream, A, to);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "cone_bp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;

@@ -266,7 +266,7 @@ struct RampOperand {
     int nC, Np, Kp;
 };
 // rows x nC inputs (FP32 or BF16 storage), optionally times w[r % wrows][c], filtered
-// into FP32 outputs; pairs: written as (q[c], q[min(c+1, nC-1)]) float2 per column.
+// into FP32 outputs.
 struct RowWeights {
     const float *w = nullptr;   // row r uses w[r % wrows][c]
     int wrows = 1;
@@ -274,9 +274,9 @@ struct RowWeights {
     int w2div = 1, w2rows = 1;
 };
 fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
-                             const RowWeights &rw, hipStream_t stream, bool pairs);
-// q: filtered projections, plain [..][nCols] FP32 or (pairs) the column-pair layout.
-fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q, bool pairs,
+                             const RowWeights &rw, hipStream_t stream);
+// q: filtered projections, FP32 [n][nViews][nRows][nCols].
+fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, const float *q,
                            void *vol, fbct_dtype to, int n, hipStream_t stream);
 fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in,
                                     float *out, int64_t rows, hipStream_t stream);

@@ -59,7 +59,7 @@ __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uin
 // so the 16 rows a fragment read touches fall in distinct banks.
 constexpr int kBM = 128, kBN = 128, kBK = 32, kKS = 40;
 
-template <typename TI, bool kW, bool kPairs>
+template <typename TI, bool kW>
 __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 {
     constexpr bool kExactA = sizeof(TI) == 2 && !kW;
@@ -200,29 +200,21 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
             for (int j = 0; j < 4; ++j) {
                 const long long row = r0 + wr * 64 + 16 * i + 4 * (lane >> 4) + j;
                 if (row >= A.rows) continue;
-                if constexpr (kPairs) {
-                    float *o = A.out + 2 * (row * A.nC + col);
-                    o[0] = acc[i][t][j];
-                    if (col > 0) o[-1] = acc[i][t][j];
-                    if (col == A.nC - 1) o[1] = acc[i][t][j];
-                } else {
-                    A.out[row * A.nC + col] = acc[i][t][j];
-                }
+                A.out[row * A.nC + col] = acc[i][t][j];
             }
     }
 }
 
 template <typename TI, bool kW>
-void launch_gemm_p(bool pairs, dim3 grid, hipStream_t stream, const MfmaArgs &A)
+void launch_gemm_p(dim3 grid, hipStream_t stream, const MfmaArgs &A)
 {
-    if (pairs) hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, true>), grid, dim3(256), 0, stream, A);
-    else hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, false>), grid, dim3(256), 0, stream, A);
+    hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW>), grid, dim3(256), 0, stream, A);
 }
 
 }  // namespace
 
 fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti, float *out, int64_t rows,
-                             const RowWeights &rw, hipStream_t stream, bool pairs)
+                             const RowWeights &rw, hipStream_t stream)
 {
     if (rows <= 0) return FBCT_OK;
     if ((rw.w && rw.wrows <= 0) || (rw.w2 && (rw.w2div <= 0 || rw.w2rows <= 0)))
@@ -247,11 +239,11 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
     dim3 grid((unsigned)gx, (H.nC + kBN - 1) / kBN);
     if (ti == FBCT_BF16) {
-        if (w) launch_gemm_p<uint16_t, true>(pairs, grid, stream, A);
-        else launch_gemm_p<uint16_t, false>(pairs, grid, stream, A);
+        if (w) launch_gemm_p<uint16_t, true>(grid, stream, A);
+        else launch_gemm_p<uint16_t, false>(grid, stream, A);
     } else {
-        if (w) launch_gemm_p<float, true>(pairs, grid, stream, A);
-        else launch_gemm_p<float, false>(pairs, grid, stream, A);
+        if (w) launch_gemm_p<float, true>(grid, stream, A);
+        else launch_gemm_p<float, false>(grid, stream, A);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_gemm_kernel launch: %s", hipGetErrorString(e));
@@ -262,7 +254,7 @@ fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, c
                                     int64_t rows, hipStream_t stream)
 {
     RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp};
-    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, RowWeights{}, stream, false);
+    return launch_ramp_gemm(H, in, FBCT_F32, out, rows, RowWeights{}, stream);
 }
 
 }  // namespace fbct

@@ -316,7 +316,7 @@ fbct_status fbct_cone_plan_create(const fbct_cone_geom *geom, const float *angle
 fbct_status fbct_cone_plan_destroy(fbct_cone_plan *plan);
 
 /* Workspace bytes for fbct_cone_fdk on n volumes: the filtered projections as FP32
- * column pairs (q[c], q[c+1]), 8 bytes per detector sample. */
+ * rows, 4 bytes per detector sample. */
 size_t fbct_cone_workspace_size(const fbct_cone_plan *plan, int n);
 
 /* Row filter alone (pre-weight + ramp): proj (FP32 or BF16 storage) -> FP32. */
