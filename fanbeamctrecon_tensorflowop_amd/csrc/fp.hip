@@ -9,10 +9,13 @@
 //   * a workgroup owns a bundle of 256/G adjacent rays of one view and marches them
 //     slab by slab along the axis the bundle is closest to;  for every slab it stages
 //     the exact bounding box of the taps its samples touch into LDS (clamped to the
-//     image edge, which reproduces the clamp-to-edge texture address mode) and
-//     interpolates from LDS with exact FP32 bilinear weights;
-//   * G lanes share a ray, each keeps its own FP32 partial sum across all
-//     slabs; one cross-lane reduction per ray at the end and one coalesced store.
+//     image edge, which reproduces the clamp-to-edge texture address mode);
+//   * G lanes share a ray, each takes a contiguous share of the ray's samples in the
+//     slab and walks it texel cell by texel cell: the bilinear weights of a cell's
+//     samples are summed in closed form and its four taps are read once (a ray spends
+//     6-8 samples in a cell at the 1/8-pixel step);
+//   * FP32 partial sums per slab, FP64 across slabs; one cross-lane reduction per ray at
+//     the end and one coalesced store.
 // A sample belongs to the slab containing its major coordinate, decided from the same
 // FP32 position the interpolation uses, so every sample is counted exactly once.
 
@@ -135,6 +138,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
         dm = xmajor ? g.w : g.z;
     }
     const bool asc = da >= 0.0f;
+    // cell walk constants: reciprocal steps (a zero step never crosses: a large finite
+    // value), the boundary offset of a cell in the walking direction, da * dm
+    const float ia = da != 0.0f ? 1.0f / da : 1.0e30f;
+    const float im = dm != 0.0f ? 1.0f / dm : 1.0e30f;
+    const float oa = da >= 0.0f ? 1.0f : 0.0f;
+    const float om = dm >= 0.0f ? 1.0f : 0.0f;
+    const float dadm = da * dm;
     const TI *img[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -212,35 +222,44 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 
         // byte offset of tap (fa, fm) = (fm - mlo) * kRowB + (fa - alo) * kColB, exact in FP32
         const float obase = tbase - ((float)mlo * kRowB + (float)alo * kColB);
-        const float kend = (float)kb;
+        // The slab's samples [ka, kb) are split into G contiguous shares, one per lane.  All
+        // samples inside one texel cell (floor a, floor m) read the same four taps, so a lane
+        // walks its share cell by cell: the n samples of a cell have fractions ta + j da and
+        // tm + j dm (j < n), their bilinear weights are summed in closed form and the taps
+        // are read and weighted once per cell.  A sample on a cell boundary may be counted
+        // in either cell: the bilinear interpolant is continuous there.
+        const int len = (kb - ka + G - 1) / G;
+        const int kl = ka + lane * len;
+        const float kend = (float)min(kl + len, kb);
         float acc[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] = 0.0f;
-        for (float kf = (float)(ka + lane); kf < kend; kf += (float)G) {
+        for (float kf = (float)kl; kf < kend;) {
             const float a = fmaf(kf, da, a0);
             const float m = fmaf(kf, dm, m0);
             const float fa = floorf(a), fm = floorf(m);
             const float ta = a - fa, tm = m - fm;
+            // samples before the next boundary along each axis: floor(distance / step) + 1
+            const float ja = floorf((fa + oa - a) * ia);
+            const float jm = floorf((fm + om - m) * im);
+            const float n = fminf(fminf(ja, jm) + 1.0f, kend - kf);
+            const float t1 = 0.5f * fmaf(n, n, -n);                        // sum of j
+            const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);      // sum of j^2
+            const float sa = fmaf(da, t1, n * ta);                          // sum of a-fractions
+            const float sm = fmaf(dm, t1, n * tm);                          // sum of m-fractions
+            const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, dm, tm * da), t1, dadm * t2));
+            const float w11 = sam, w01 = sa - sam, w10 = sm - sam, w00 = (n - sa) - w10;
             const unsigned o = (unsigned)fmaf(fm, kRowB, fmaf(fa, kColB, obase));
             const TapT t00 = *reinterpret_cast<LdsTap *>((size_t)o);
             const TapT t01 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)kColB));
             const TapT t10 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)kRowB));
             const TapT t11 = *reinterpret_cast<LdsTap *>((size_t)(o + (unsigned)(kRowB + kColB)));
-            if (S == 1) {
-                const float v00 = lane_of<S>(t00, 0), v01 = lane_of<S>(t01, 0);
-                const float v10 = lane_of<S>(t10, 0), v11 = lane_of<S>(t11, 0);
-                const float top = fmaf(ta, v01 - v00, v00);
-                const float bot = fmaf(ta, v11 - v10, v10);
-                acc[0] += fmaf(tm, bot - top, top);
-            } else {
-                const float ua = 1.0f - ta, um = 1.0f - tm;
-                const float w00 = ua * um, w01 = ta * um, w10 = ua * tm, w11 = ta * tm;
 #pragma unroll
-                for (int s = 0; s < S; ++s)
-                    acc[s] = fmaf(w00, lane_of<S>(t00, s),
-                                  fmaf(w01, lane_of<S>(t01, s),
-                                       fmaf(w10, lane_of<S>(t10, s), fmaf(w11, lane_of<S>(t11, s), acc[s]))));
-            }
+            for (int s = 0; s < S; ++s)
+                acc[s] = fmaf(w00, lane_of<S>(t00, s),
+                              fmaf(w01, lane_of<S>(t01, s),
+                                   fmaf(w10, lane_of<S>(t10, s), fmaf(w11, lane_of<S>(t11, s), acc[s]))));
+            kf += n;
         }
 #pragma unroll
         for (int s = 0; s < S; ++s) accd[s] += (double)acc[s];
