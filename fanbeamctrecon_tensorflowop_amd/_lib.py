@@ -92,6 +92,8 @@ SIGNATURES = [
     ("fbct_plan_geom", _I, [_P, ctypes.POINTER(FanGeom)]),
     ("fbct_plan_fp_samples", ctypes.c_int64, [_P]),
     ("fbct_plan_ray_table", _I, [_P, ctypes.POINTER(ctypes.c_int32), _F]),
+    ("fbct_plan_fp_bundles", _I, [_P, _I, ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p]),
     ("fbct_plan_ramp_taps", _I, [_P, ctypes.POINTER(ctypes.c_double)]),
     ("fbct_plan_upload", _I, [_P]),
     ("fbct_plan_parker_weights", _I, [_P, _F]),

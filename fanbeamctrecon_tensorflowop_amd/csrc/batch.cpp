@@ -58,6 +58,8 @@ fbct_status batch_device(fbct_batch *b, const BatchDevice **out)
         f.ray_n = t->ray_n;
         for (int k = 0; k < kFpNumCfg; ++k) {
             f.major[k] = t->fp_major[k];
+            f.slabs[k] = t->fp_slabs[k];
+            f.rows[k] = t->fp_rows[k];
             f.nb[k] = p->d.fpb[k].nb;
             f.cap[k] = p->d.fpb[k].cap;
             f.nslab[k] = p->d.fpb[k].nslab;

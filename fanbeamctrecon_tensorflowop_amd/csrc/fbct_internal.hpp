@@ -65,6 +65,8 @@ struct FpSliceDesc {
     const float4 *ray_geo;
     const int32_t *ray_n;
     const uint8_t *major[kFpNumCfg];
+    const short2 *slabs[kFpNumCfg];
+    const short2 *rows[kFpNumCfg];
     long long img_off, sino_off;
     int nX, nY, nViews, nCols;
     int nb[kFpNumCfg], cap[kFpNumCfg], nslab[kFpNumCfg];
@@ -88,6 +90,8 @@ struct DeviceTables {
     float4 *ray_geo = nullptr;     // [nViews*nCols] (u0, w0, du, dw)
     float2 *ray_inv = nullptr;     // [nViews*nCols] (1/du, 1/dw)
     uint8_t *fp_major[kFpNumCfg] = {};  // per bundle config: [nViews*nb] 1 = x-major
+    short2 *fp_slabs[kFpNumCfg] = {};   // per bundle config: [nViews*nb] first and last slab
+    short2 *fp_rows[kFpNumCfg] = {};    // per bundle config: [nViews*nb*nslab] tile rows [lo, hi]
     float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
     double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
     double *ramp = nullptr;        // [2*nCols-1]
@@ -128,6 +132,8 @@ struct fbct_plan {
     std::vector<float> ray_geo;     // 4 per ray
     std::vector<float> ray_inv;     // 2 per ray
     std::vector<uint8_t> fp_major[fbct::kFpNumCfg];
+    std::vector<int16_t> fp_slabs[fbct::kFpNumCfg];  // 2 per bundle
+    std::vector<int16_t> fp_rows[fbct::kFpNumCfg];   // 2 per (bundle, slab)
     std::vector<float> view_cs;     // 2 per view
     std::vector<double> view_csd;   // 2 per view
     std::vector<double> ramp;       // 2*nCols-1

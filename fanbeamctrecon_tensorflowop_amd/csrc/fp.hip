@@ -42,6 +42,8 @@ struct FpArgs {
     const float4 *ray_geo;
     const int32_t *ray_n;
     const uint8_t *major;
+    const short2 *slabs;   // [nViews*nb] first and last slab with samples
+    const short2 *rows;    // [nViews*nb*nslab] tile rows [lo, hi] (lo > hi: empty)
     long long img_stride;
     long long sino_stride;
     int nX, nY, nViews, nCols, nb, nSlices;
@@ -66,8 +68,9 @@ __device__ __forceinline__ float lane_of(const typename Taps<S>::T &t, int s)
 
 // Smallest k in [0, N] with P(k) true, where P(k) = (a0 + k da >= t) for da >= 0 and
 // (a0 + k da < t) for da < 0, evaluated with the same fmaf the samples use (monotone
-// in k, so P is false...false true...true).
-__device__ __forceinline__ int slab_boundary(float a0, float da, int N, float t)
+// in k, so P is false...false true...true).  ia = 1/da estimates k; the two scans make
+// it exact.
+__device__ __forceinline__ int slab_boundary(float a0, float da, float ia, int N, float t)
 {
     if (N <= 0) return 0;
     const bool asc = da >= 0.0f;
@@ -75,7 +78,7 @@ __device__ __forceinline__ int slab_boundary(float a0, float da, int N, float t)
     if (da == 0.0f) {
         est = (a0 >= t) ? 0.0f : (float)N;
     } else {
-        const float q = (t - a0) / da;
+        const float q = (t - a0) * ia;
         est = asc ? ceilf(q) : floorf(q) + 1.0f;
     }
     est = fminf(fmaxf(est, 0.0f), (float)N);
@@ -89,23 +92,11 @@ __device__ __forceinline__ int slab_boundary(float a0, float da, int N, float t)
     return k;
 }
 
-__device__ __forceinline__ int wave_min(int v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max(int v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    return v;
-}
-
 // S slices that share the geometry are projected together: the LDS tile interleaves
 // them ([minor][major][slice]) so one sample position, its weights and its LDS
 // address serve S slices, with S-wide LDS reads.
 // Dynamic LDS of the FP kernels: the slab tile at address 0 (so tap offsets need no
-// base add), followed by the per-wave min/max scratch red[2][2][waves].
+// base add).
 extern __shared__ float fp_lds[];
 
 // Bundle b of view v for slices [s0, s0+S): every thread of the workgroup calls it with
@@ -117,8 +108,6 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     constexpr int kSlab = kFpCfgSlab[fp_cfg(G)];
     constexpr int kPitch = kSlab + 1;
     float *tile = fp_lds;
-    int(*red)[2][kFpThreads / 64] =
-        reinterpret_cast<int(*)[2][kFpThreads / 64]>(tile + (size_t)kPitch * A.cap * S);
 
     const int tid = threadIdx.x;
     constexpr int kRays = kFpThreads / G;
@@ -156,8 +145,11 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     double accd[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) accd[s] = 0.0;
-    int kprev = slab_boundary(a0, da, N, -1.0f);
-    const int wave = tid / 64;
+    // slabs of the bundle with samples, and per slab the tile's minor rows (plan tables)
+    const size_t vb = (size_t)v * A.nb + b;
+    const short2 srange = A.slabs[vb];
+    const short2 *rows = A.rows + vb * A.nslab;
+    int kprev = slab_boundary(a0, da, ia, N, (float)(srange.x * kSlab - 1));
     // byte strides of the interleaved tile
     constexpr float kColB = 4.0f * S;
     constexpr float kRowB = 4.0f * S * kPitch;
@@ -165,39 +157,22 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     const float tbase = (float)(unsigned)(size_t)((__attribute__((address_space(3))) float *)(tile));
     using LdsTap = __attribute__((address_space(3))) const TapT;
 
-    for (int sl = 0; sl < A.nslab; ++sl) {
+    for (int sl = srange.x; sl <= srange.y; ++sl) {
         const int alo = sl * kSlab - 1;
-        const int knext = slab_boundary(a0, da, N, (float)(alo + kSlab));
+        const int knext = slab_boundary(a0, da, ia, N, (float)(alo + kSlab));
         const int ka = asc ? kprev : knext;
         const int kb = asc ? knext : kprev;
         kprev = knext;
 
-        int mn = INT_MAX, mx = INT_MIN;
-        if (ka < kb) {
-            const float m1 = fmaf((float)ka, dm, m0);
-            const float m2 = fmaf((float)(kb - 1), dm, m0);
-            mn = (int)floorf(fminf(m1, m2));
-            mx = (int)floorf(fmaxf(m1, m2)) + 1;
-        }
-        mn = wave_min(mn);
-        mx = wave_max(mx);
-        if ((tid & 63) == 0) {
-            red[sl & 1][0][wave] = mn;
-            red[sl & 1][1][wave] = mx;
-        }
-        __syncthreads();  // reductions visible; previous slab's reads of `tile` done
-        int mlo = red[sl & 1][0][0], mhi = red[sl & 1][1][0];
-#pragma unroll
-        for (int w = 1; w < kFpThreads / 64; ++w) {
-            mlo = min(mlo, red[sl & 1][0][w]);
-            mhi = max(mhi, red[sl & 1][1][w]);
-        }
-        if (mlo > mhi) continue;  // no samples of this bundle in the slab (uniform)
+        const short2 rr = rows[sl];  // uniform
+        const int mlo = rr.x, mhi = rr.y;
+        if (mlo > mhi) continue;  // no samples of this bundle in the slab
         const int M = mhi - mlo + 1;
         if (M > A.cap) {  // cannot happen for tables built by the plan; never read past LDS
             if (tid == 0) atomicOr(err, 1);
             continue;
         }
+        __syncthreads();  // the previous slab's reads of `tile` are done
 
         // Stage taps [alo, alo+kSlab] x [mlo, mhi] of the S slices with clamp-to-edge
         // addressing.  Global loads run along the image row (i) in both orientations.
@@ -307,6 +282,8 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.ray_geo = d.ray_geo;
         A.ray_n = d.ray_n;
         A.major = d.major[cfg];
+        A.slabs = d.slabs[cfg];
+        A.rows = d.rows[cfg];
         A.img_stride = 0;
         A.sino_stride = 0;
         A.nX = d.nX;
@@ -320,7 +297,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.scale = d.scale;
         const int local = item - prefix[lo];
         fp_bundle<1, G, TI, TO>(A, local % A.nb, local / A.nb, 0, err);
-        __syncthreads();  // the next item reuses the LDS tile and reduction scratch
+        __syncthreads();  // the next item reuses the LDS tile
     }
 }
 
@@ -391,7 +368,6 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     // Slices per workgroup S and bundle configuration: share each sample's position and
     // weights across 4 slices with 32-ray bundles (smaller tile), else 2 or 1 slices with
     // 64-ray bundles, while the grid keeps >= 2048 workgroups and the tile <= 80 KB.
-    const size_t red_bytes = 2 * 2 * (kFpThreads / 64) * sizeof(int);
     const size_t kMaxTile = 80 * 1024;
     auto tile_bytes = [&](int cfg, int S) { return (size_t)fp_pitch(cfg) * p.d.fpb[cfg].cap * sizeof(float) * S; };
     auto grid_wgs = [&](int cfg, int S) { return (long long)p.d.fpb[cfg].nb * p.g.nViews * ((n + S - 1) / S); };
@@ -407,10 +383,12 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     }
     const FpBundleCfg &bc = p.d.fpb[cfg];
     A.major = t.fp_major[cfg];
+    A.slabs = t.fp_slabs[cfg];
+    A.rows = t.fp_rows[cfg];
     A.nb = bc.nb;
     A.cap = bc.cap;
     A.nslab = bc.nslab;
-    const size_t lds = tile_bytes(cfg, S) + red_bytes;
+    const size_t lds = tile_bytes(cfg, S);
     if (lds > 160 * 1024)
         return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds);
     dim3 grid(bc.nb, p.g.nViews, (n + S - 1) / S);
@@ -450,7 +428,7 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
     int *err = nullptr;
     if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
     hipError_t e;
-    const size_t lds = (size_t)fp_pitch(cfg) * b.fp_cap_max[cfg] * sizeof(float) + 2 * 2 * (kFpThreads / 64) * sizeof(int);
+    const size_t lds = (size_t)fp_pitch(cfg) * b.fp_cap_max[cfg] * sizeof(float);
     if (lds > 64 * 1024)
         return fail(FBCT_ERR_UNSUPPORTED, "fbct_batch_fp: slab tile of %zu bytes exceeds 64 KiB", lds);
     const long long wgs = std::min<long long>(total, (long long)b.shape.fp_wgs_per_cu * bd.num_cu);

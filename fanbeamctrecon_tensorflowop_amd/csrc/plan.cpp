@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "fbct_internal.hpp"
 
@@ -102,6 +103,21 @@ static fbct_status validate(const fbct_fan_geom &g)
     if (g.nX > 16384 || g.nY > 16384 || g.nCols > 65536)
         return fail(FBCT_ERR_INVALID_GEOMETRY, "image side or detector width too large");
     return FBCT_OK;
+}
+
+// Runs fn(v_lo, v_hi) over [0, nV) on up to 16 threads (disjoint view ranges).
+template <typename Fn>
+static void parallel_views(int nV, Fn &&fn)
+{
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = static_cast<int>(std::min<unsigned>(hw, static_cast<unsigned>(std::max(1, nV / 32))));
+    if (nt <= 1) {
+        fn(0, nV);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(fn, nV * t / nt, nV * (t + 1) / nt);
+    for (std::thread &x : th) x.join();
 }
 
 // ------------------------------------------------------------ FP ray table
@@ -262,7 +278,6 @@ static void build_tables(fbct_plan &p)
     const fbct_fan_geom &g = p.g;
     const int nV = g.nViews, nC = g.nCols;
     const double r = static_cast<double>(g.rFOV) / static_cast<double>(g.dx);
-    const double Rp = static_cast<double>(g.R) / static_cast<double>(g.dx);
 
     p.ray_n.assign(static_cast<size_t>(nV) * nC, 0);
     p.ray_geo.assign(static_cast<size_t>(nV) * nC * 4, 0.0f);
@@ -294,9 +309,14 @@ static void build_tables(fbct_plan &p)
     p.fp_samples = samples;
 
     // FP bundles: rays [b*rays, (b+1)*rays) of a view march slab by slab along the
-    // axis their central ray is closest to.  cap bounds the minor extent of the taps
-    // of one slab, from the ray lines in FP64 plus a margin that covers the FP32 sample
-    // positions; the kernel derives the exact extent from its own positions.
+    // axis their central ray is closest to; slab sl holds the samples whose major
+    // coordinate lies in [sl*slab - 1, (sl+1)*slab - 1) and stages taps [alo, alo+slab]
+    // along it.  Per (bundle, slab) the tile's minor rows [lo, hi] bound the taps of the
+    // bundle's samples in the slab: each ray's segment in the slab, in FP64 on the FP32
+    // ray parameters the kernel uses, widened by one row on each side (which covers the
+    // kernel's FP32 sample positions).  Per bundle: the first and last slab with samples
+    // (a slab outside is empty for every ray, also in FP32: the range is widened by 0.01
+    // pixel).  cap is the largest row count of a tile.
     Derived &d = p.d;
     d.fp_scale = static_cast<float>(0.125 * static_cast<double>(g.dx));
     d.two_r = static_cast<float>(2.0 * r);
@@ -308,40 +328,79 @@ static void build_tables(fbct_plan &p)
         bc.slab = slab;
         bc.nslab = static_cast<int>(std::floor((2.0 * r + 1.0) / slab)) + 1;
         bc.nb = (nC + rays - 1) / rays;
+        const int nb = bc.nb, nslab = bc.nslab;
         std::vector<uint8_t> &major = p.fp_major[cfg];
-        major.assign(static_cast<size_t>(nV) * bc.nb, 1);
-        double cap = 0.0;
-        for (int v = 0; v < nV; ++v) {
-            const double beta = static_cast<double>(p.angles[v]);
-            const double su = Rp * std::cos(beta) + r - 0.5;  // source, texel coordinates
-            const double sw = Rp * std::sin(beta) + r - 0.5;
-            for (int b = 0; b < bc.nb; ++b) {
-                const int c0 = b * rays;
-                const int c1 = std::min(nC, c0 + rays) - 1;
-                const double gmid = static_cast<double>(g.dLeft) + (0.5 * (c0 + c1) + 0.5) * static_cast<double>(g.dCol);
-                const double thm = beta + gmid;
-                const bool xmajor = std::fabs(std::cos(thm)) >= std::fabs(std::sin(thm));
-                major[static_cast<size_t>(v) * bc.nb + b] = xmajor ? 1 : 0;
-                const double sa = xmajor ? su : sw, sm = xmajor ? sw : su;
-                double slope_max = 0.0, lo_a = 1e300, hi_a = -1e300, lo_b = 1e300, hi_b = -1e300;
-                for (int c : {c0, c1}) {
-                    const double th = beta + static_cast<double>(g.dLeft) + (c + 0.5) * static_cast<double>(g.dCol);
-                    const double da = xmajor ? std::cos(th) : std::sin(th);
-                    const double dm = xmajor ? std::sin(th) : std::cos(th);
-                    const double slope = dm / da;
-                    slope_max = std::max(slope_max, std::fabs(slope));
-                    const double ma = sm + (-1.0 - sa) * slope;
-                    const double mb = sm + (2.0 * r + 1.0 - sa) * slope;
-                    lo_a = std::min(lo_a, ma);
-                    hi_a = std::max(hi_a, ma);
-                    lo_b = std::min(lo_b, mb);
-                    hi_b = std::max(hi_b, mb);
+        major.assign(static_cast<size_t>(nV) * nb, 1);
+        std::vector<int16_t> &slabs = p.fp_slabs[cfg];
+        std::vector<int16_t> &rows = p.fp_rows[cfg];
+        slabs.assign(static_cast<size_t>(nV) * nb * 2, 0);
+        rows.assign(static_cast<size_t>(nV) * nb * nslab * 2, 0);
+        std::vector<int> cap_of(nV, 0);
+        auto views = [&](int v_lo, int v_hi) {
+            std::vector<double> lo(nslab), hi(nslab);
+            for (int v = v_lo; v < v_hi; ++v) {
+                const double beta = static_cast<double>(p.angles[v]);
+                for (int b = 0; b < nb; ++b) {
+                    const int c0 = b * rays;
+                    const int c1 = std::min(nC, c0 + rays) - 1;
+                    const double gmid = static_cast<double>(g.dLeft) + (0.5 * (c0 + c1) + 0.5) * static_cast<double>(g.dCol);
+                    const double thm = beta + gmid;
+                    const bool xmajor = std::fabs(std::cos(thm)) >= std::fabs(std::sin(thm));
+                    const size_t vb = static_cast<size_t>(v) * nb + b;
+                    major[vb] = xmajor ? 1 : 0;
+                    std::fill(lo.begin(), lo.end(), 1e300);
+                    std::fill(hi.begin(), hi.end(), -1e300);
+                    int s_first = nslab, s_last = -1;
+                    for (int c = c0; c <= c1; ++c) {
+                        const size_t k = static_cast<size_t>(v) * nC + c;
+                        const int n = p.ray_n[k];
+                        if (n <= 0) continue;
+                        const double u0 = p.ray_geo[4 * k + 0], w0 = p.ray_geo[4 * k + 1];
+                        const double du = p.ray_geo[4 * k + 2], dw = p.ray_geo[4 * k + 3];
+                        const double a0 = xmajor ? u0 : w0, da = xmajor ? du : dw;
+                        const double m0 = xmajor ? w0 : u0, dm = xmajor ? dw : du;
+                        const double a1 = a0 + (n - 1) * da, m1 = m0 + (n - 1) * dm;
+                        const double amin = std::min(a0, a1) - 0.01, amax = std::max(a0, a1) + 0.01;
+                        // minor coordinate at major coordinate e (a zero major step: the whole ray)
+                        auto m_at = [&](double e, bool upper) {
+                            if (da == 0.0) return upper ? std::max(m0, m1) : std::min(m0, m1);
+                            return m0 + (e - a0) * (dm / da);
+                        };
+                        const int sl0 = std::max(0, static_cast<int>(std::floor((amin + 1.0) / slab)));
+                        const int sl1 = std::min(nslab - 1, static_cast<int>(std::floor((amax + 1.0) / slab)));
+                        for (int sl = sl0; sl <= sl1; ++sl) {
+                            const double e0 = std::max(amin, sl * static_cast<double>(slab) - 1.0);
+                            const double e1 = std::min(amax, (sl + 1) * static_cast<double>(slab) - 1.0);
+                            if (e0 > e1) continue;
+                            const double mA = m_at(e0, false), mB = m_at(e1, true);
+                            lo[sl] = std::min(lo[sl], std::min(mA, mB));
+                            hi[sl] = std::max(hi[sl], std::max(mA, mB));
+                            s_first = std::min(s_first, sl);
+                            s_last = std::max(s_last, sl);
+                        }
+                    }
+                    slabs[2 * vb + 0] = static_cast<int16_t>(s_first);
+                    slabs[2 * vb + 1] = static_cast<int16_t>(s_last);
+                    for (int sl = 0; sl < nslab; ++sl) {
+                        int16_t *e = &rows[2 * (vb * nslab + sl)];
+                        if (hi[sl] < lo[sl]) {
+                            e[0] = 1;  // empty: lo > hi
+                            e[1] = 0;
+                            continue;
+                        }
+                        const int rlo = static_cast<int>(std::floor(lo[sl])) - 1;
+                        const int rhi = static_cast<int>(std::floor(hi[sl])) + 2;
+                        e[0] = static_cast<int16_t>(rlo);
+                        e[1] = static_cast<int16_t>(rhi);
+                        cap_of[v] = std::max(cap_of[v], rhi - rlo + 1);
+                    }
                 }
-                const double spread = std::max(hi_a - lo_a, hi_b - lo_b);
-                cap = std::max(cap, spread + slab * slope_max + 4.0);
             }
-        }
-        bc.cap = static_cast<int>(std::ceil(std::min(cap, std::ceil(2.0 * r) + 3.0)));
+        };
+        parallel_views(nV, views);
+        int cap = 1;
+        for (int c : cap_of) cap = std::max(cap, c);
+        bc.cap = cap;
     }
 
     // BP: xi_raw = (atan(t/U) - dLeft)/dCol + dShift  (BP/transform.cu:85).
@@ -430,7 +489,11 @@ static void free_tables(DeviceTables &t)
     if (t.device >= 0) (void)hipSetDevice(t.device);
     (void)hipFree(t.ray_n);
     (void)hipFree(t.ray_geo);
-    for (int cfg = 0; cfg < kFpNumCfg; ++cfg) (void)hipFree(t.fp_major[cfg]);
+    for (int cfg = 0; cfg < kFpNumCfg; ++cfg) {
+        (void)hipFree(t.fp_major[cfg]);
+        (void)hipFree(t.fp_slabs[cfg]);
+        (void)hipFree(t.fp_rows[cfg]);
+    }
     (void)hipFree(t.ray_inv);
     (void)hipFree(t.view_cs);
     (void)hipFree(t.view_csd);
@@ -480,6 +543,10 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
         (s = upload(&t.ray_geo, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.fp_major[0], plan->fp_major[0].data(), plan->fp_major[0].size())) != FBCT_OK ||
         (s = upload(&t.fp_major[1], plan->fp_major[1].data(), plan->fp_major[1].size())) != FBCT_OK ||
+        (s = upload(&t.fp_slabs[0], plan->fp_slabs[0].data(), plan->fp_slabs[0].size() * sizeof(int16_t))) != FBCT_OK ||
+        (s = upload(&t.fp_slabs[1], plan->fp_slabs[1].data(), plan->fp_slabs[1].size() * sizeof(int16_t))) != FBCT_OK ||
+        (s = upload(&t.fp_rows[0], plan->fp_rows[0].data(), plan->fp_rows[0].size() * sizeof(int16_t))) != FBCT_OK ||
+        (s = upload(&t.fp_rows[1], plan->fp_rows[1].data(), plan->fp_rows[1].size() * sizeof(int16_t))) != FBCT_OK ||
         (s = upload(&t.ray_inv, plan->ray_inv.data(), plan->ray_inv.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_csd, plan->view_csd.data(), plan->view_csd.size() * sizeof(double))) != FBCT_OK ||
@@ -593,6 +660,25 @@ fbct_status fbct_plan_ray_table(const fbct_plan *plan, int32_t *n_out, float *ge
     if (!plan) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null plan");
     if (n_out) std::memcpy(n_out, plan->ray_n.data(), plan->ray_n.size() * sizeof(int32_t));
     if (geo_out) std::memcpy(geo_out, plan->ray_geo.data(), plan->ray_geo.size() * sizeof(float));
+    return FBCT_OK;
+}
+
+fbct_status fbct_plan_fp_bundles(const fbct_plan *plan, int cfg, int32_t *info_out, uint8_t *major_out,
+                                 int16_t *slabs_out, int16_t *rows_out)
+{
+    if (!plan) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "null plan");
+    if (cfg < 0 || cfg >= fbct::kFpNumCfg) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "bundle config %d", cfg);
+    const fbct::FpBundleCfg &bc = plan->d.fpb[cfg];
+    if (info_out) {
+        info_out[0] = bc.rays;
+        info_out[1] = bc.slab;
+        info_out[2] = bc.nslab;
+        info_out[3] = bc.nb;
+        info_out[4] = bc.cap;
+    }
+    if (major_out) std::memcpy(major_out, plan->fp_major[cfg].data(), plan->fp_major[cfg].size());
+    if (slabs_out) std::memcpy(slabs_out, plan->fp_slabs[cfg].data(), plan->fp_slabs[cfg].size() * sizeof(int16_t));
+    if (rows_out) std::memcpy(rows_out, plan->fp_rows[cfg].data(), plan->fp_rows[cfg].size() * sizeof(int16_t));
     return FBCT_OK;
 }
 

@@ -96,6 +96,22 @@ class FanBeamPlan:
             self._h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _fptr(geo)), "fbct_plan_ray_table")
         return n, geo
 
+    def fp_bundles(self, cfg: int):
+        """FP bundle tables of configuration cfg: (info dict, major [nViews, nb] uint8,
+        slabs [nViews, nb, 2] int16 = first and last slab with samples, rows
+        [nViews, nb, nslab, 2] int16 = staged tile rows [lo, hi])."""
+        lib = _lib.load()
+        info = np.zeros(5, dtype=np.int32)
+        _lib.check(lib.fbct_plan_fp_bundles(self._h, cfg, info.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            None, None, None), "fbct_plan_fp_bundles")
+        rays, slab, nslab, nb, cap = (int(v) for v in info)
+        major = np.zeros((self.n_views, nb), dtype=np.uint8)
+        slabs = np.zeros((self.n_views, nb, 2), dtype=np.int16)
+        rows = np.zeros((self.n_views, nb, nslab, 2), dtype=np.int16)
+        _lib.check(lib.fbct_plan_fp_bundles(self._h, cfg, None, major.ctypes.data, slabs.ctypes.data,
+                                            rows.ctypes.data), "fbct_plan_fp_bundles")
+        return dict(rays=rays, slab=slab, nslab=nslab, nb=nb, cap=cap), major, slabs, rows
+
     def ramp_taps(self) -> np.ndarray:
         h = np.zeros(2 * self.n_cols - 1, dtype=np.float64)
         _lib.check(_lib.load().fbct_plan_ramp_taps(self._h, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),

@@ -120,6 +120,15 @@ int64_t fbct_plan_fp_samples(const fbct_plan *plan);
  * coordinates of sample 0 (the FP32 values the kernels use).  Either may be NULL. */
 fbct_status fbct_plan_ray_table(const fbct_plan *plan, int32_t *n_out, float *geo_out);
 
+/* Host copy of the FP bundle tables of bundle configuration cfg (0: 64 rays per
+ * bundle, 1: 32): info_out[5] = (rays, slab, nslab, nb, cap); major_out[nViews*nb]
+ * (1 = bundle marches along x); slabs_out[nViews*nb*2] = first and last slab holding
+ * samples; rows_out[nViews*nb*nslab*2] = staged tile rows [lo, hi] per slab (lo > hi:
+ * no samples).  Any output may be NULL.  No counterpart in the reference (whose FP
+ * samples a texture per ray); it lets tests check the tables on the host. */
+fbct_status fbct_plan_fp_bundles(const fbct_plan *plan, int cfg, int32_t *info_out, uint8_t *major_out,
+                                 int16_t *slabs_out, int16_t *rows_out);
+
 /* Host copy of the ramp taps h[n], n in [-(nCols-1), nCols-1] (2*nCols-1 doubles). */
 fbct_status fbct_plan_ramp_taps(const fbct_plan *plan, double *taps_out);
 

@@ -142,6 +142,46 @@ def test_ray_table_matches_oracle(oracle_mod):
                 np.testing.assert_array_equal(geo[v, c], np.asarray(vals, dtype=np.float32))
 
 
+def _fp_bundle_geometries():
+    yield "small", workloads.small_scan(), 40
+    yield "clinical", workloads.clinical_scan(), 12
+    # source close to the FOV (R = 1.3 rFOV), wide fan, off-centre detector, non-square image
+    yield "close", workloads.scan(nx=80, ny=64, rfov=40.0, sid=52.0, sdd=110.0, n_cols=200,
+                                  dcol_rad=1.6 / 200, n_views=90, dshift=3.5), None
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_fp_bundle_tables_cover_samples(cfg):
+    """Every FP sample, at the FP32 position the kernel computes (fma of k, step, start),
+    lies in a slab of its bundle's slab range, and its taps (floor m, floor m + 1) lie in
+    that slab's staged rows; no tile exceeds cap rows."""
+    for name, (p, a), n_check in _fp_bundle_geometries():
+        plan = FanBeamPlan(p, a)
+        info, major, slabs, rows = plan.fp_bundles(cfg)
+        n, geo = plan.ray_table()
+        lo, hi = rows[..., 0].astype(np.int64), rows[..., 1].astype(np.int64)
+        used = lo <= hi
+        assert (hi - lo + 1)[used].max() <= info["cap"], name
+        views = range(plan.n_views) if n_check is None else np.linspace(0, plan.n_views - 1, n_check).astype(int)
+        for v in views:
+            for b in range(info["nb"]):
+                c0, c1 = b * info["rays"], min(plan.n_cols, (b + 1) * info["rays"])
+                xm = major[v, b] == 1
+                for c in range(c0, c1):
+                    N = int(n[v, c])
+                    if N == 0:
+                        continue
+                    u0, w0, du, dw = (np.float32(t) for t in geo[v, c])
+                    a0, da, m0, dm = (u0, du, w0, dw) if xm else (w0, dw, u0, du)
+                    k = np.arange(N, dtype=np.float64)
+                    av = (k * np.float64(da) + np.float64(a0)).astype(np.float32).astype(np.float64)
+                    mv = (k * np.float64(dm) + np.float64(m0)).astype(np.float32).astype(np.float64)
+                    sl = np.floor((av + 1.0) / info["slab"]).astype(np.int64)
+                    assert sl.min() >= slabs[v, b, 0] and sl.max() <= slabs[v, b, 1], (name, v, b, c)
+                    fm = np.floor(mv).astype(np.int64)
+                    assert (lo[v, b, sl] <= fm).all() and (fm + 1 <= hi[v, b, sl]).all(), (name, v, b, c)
+
+
 def test_ramp_taps_match_oracle(oracle_mod):
     p, a = workloads.clinical_scan()
     np.testing.assert_allclose(FanBeamPlan(p, a).ramp_taps(), oracle_mod.Oracle(p, a).ramp_taps(), rtol=1e-15,
