@@ -21,6 +21,17 @@ __device__ __forceinline__ float load_f(const bf16_t *p, size_t i)
 {
     return __uint_as_float(static_cast<unsigned>(p[i]) << 16);
 }
+// Element at a 32-bit byte offset from a base held in scalar registers: the address is
+// formed by the load itself (base + zero-extended offset), no 64-bit add per element.
+__device__ __forceinline__ float load_f_at(const float *p, unsigned byte_off)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+__device__ __forceinline__ float load_f_at(const bf16_t *p, unsigned byte_off)
+{
+    const bf16_t b = *reinterpret_cast<const bf16_t *>(reinterpret_cast<const char *>(p) + byte_off);
+    return __uint_as_float(static_cast<unsigned>(b) << 16);
+}
 
 __device__ __forceinline__ bf16_t f32_to_bf16(float v)
 {

@@ -17,7 +17,7 @@
 // closed form and the samples are evaluated at exactly the FP32 positions the forward
 // kernel interpolates at.  A group of S <= 16 slices that share the geometry is
 // processed together: the per-(pixel, view, ray) weight W_c is computed once and applied
-// to S sinograms.  Sums over views accumulate in FP64.
+// to S sinograms.  Sums over blocks of 8 views accumulate in FP32, across blocks in FP64.
 
 #include <hip/hip_runtime.h>
 
@@ -50,9 +50,85 @@ struct FpaArgs {
     float far_x, far_y;     // how far samples reach beyond the last pixel centre (texels)
 };
 
-// Per-pixel accumulation over all views.  kBorder: the workgroup's tile touches the
-// image edge, so the clamp window and the unbounded outer support of border pixels
-// apply; interior tiles skip them.
+// Sample window of one pixel.  kBorder: the pixel's tile touches the image edge, so the
+// clamp window and the unbounded outer support of border pixels apply; interior tiles
+// skip them.
+struct PixelWin {
+    float xlo_c, xhi_c, ylo_c, yhi_c;  // clamp window of the sample coordinate (border plateaus)
+    float xlo_b, xhi_b, ylo_b, yhi_b;  // support of T_i in (u - i), unbounded outside border pixels
+    float rho;                          // support radius (texels)
+};
+
+template <bool kBorder>
+__device__ __forceinline__ PixelWin pixel_window(const FpaArgs &A, int i, int j)
+{
+    PixelWin w{-kBig, kBig, -kBig, kBig, -1.0f, 1.0f, -1.0f, 1.0f, 1.41421356f};
+    if (kBorder) {
+        if (i == 0) w.xlo_c = 0.0f, w.xlo_b = -kBig;
+        if (i == A.nX - 1) w.xhi_c = 0.0f, w.xhi_b = kBig;
+        if (j == 0) w.ylo_c = 0.0f, w.ylo_b = -kBig;
+        if (j == A.nY - 1) w.yhi_c = 0.0f, w.yhi_b = kBig;
+        const float ex = (i == A.nX - 1) ? fmaxf(1.0f, A.far_x) : 1.0f;
+        const float ey = (j == A.nY - 1) ? fmaxf(1.0f, A.far_y) : 1.0f;
+        w.rho = sqrtf(ex * ex + ey * ey);
+    }
+    return w;
+}
+
+// Candidate rays of a pixel in one view: columns [clo, chi] whose lines pass within rho
+// (texels) of the pixel centre (x, y) (mm).  Such rays subtend at most asin(rho dx / L)
+// from the pixel's fan angle; L >= U and asin(z) <= z (1 + z^2) for z <= 0.9.
+__device__ __forceinline__ void candidate_rays(const FpaArgs &A, float2 cs, float x, float y, float rho, int &clo,
+                                               int &chi)
+{
+    const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
+    const float t = fmaf(x, cs.y, -y * cs.x);
+    const float rU = __builtin_amdgcn_rcpf(U);
+    const float cf = fmaf(atanf(t * rU), A.c_scale, A.c_off);
+    const float z = rho * A.dx * rU;
+    const float hw = z < 0.9f ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs, 1e-3f) : 1e9f;
+    clo = (int)fmaxf(ceilf(cf - hw), 0.0f);
+    chi = (int)fminf(floorf(cf + hw), (float)(A.nCols - 1));
+}
+
+// W_c(i, j): total bilinear weight pixel (fi, fj) receives from the N samples of ray
+// (u0, w0, du, dw) = g, evaluated at the FP32 sample positions; inv = (1/du, 1/dw).
+template <bool kBorder>
+__device__ __forceinline__ float ray_weight(float4 g, float2 inv, int N, float fi, float fj, const PixelWin &w)
+{
+    const float X0 = g.x - fi, Y0 = g.y - fj;
+    const float a1 = (w.xlo_b - X0) * inv.x, a2 = (w.xhi_b - X0) * inv.x;
+    const float b1 = (w.ylo_b - Y0) * inv.y, b2 = (w.yhi_b - Y0) * inv.y;
+    const float lo = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), 0.0f);
+    const float hi = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), (float)(N - 1));
+    float wsum = 0.0f;
+    const float kend = floorf(hi);
+    auto sample = [&](float kf) {
+        const float X = fmaf(kf, g.z, X0);
+        const float Y = fmaf(kf, g.w, Y0);
+        if (kBorder) {
+            const float wx = 1.0f - fabsf(fminf(fmaxf(X, w.xlo_c), w.xhi_c));
+            const float wy = 1.0f - fabsf(fminf(fmaxf(Y, w.ylo_c), w.yhi_c));
+            wsum = fmaf(fmaxf(wx, 0.0f), fmaxf(wy, 0.0f), wsum);
+        } else {
+            // inside the support |X|, |Y| < 1 up to rounding at its two ends
+            wsum = fmaf(1.0f - fabsf(X), 1.0f - fabsf(Y), wsum);
+        }
+    };
+    float kf = ceilf(lo);
+    for (; kf + 1.0f <= kend; kf += 2.0f) {  // two samples per iteration
+        sample(kf);
+        sample(kf + 1.0f);
+    }
+    if (kf <= kend) sample(kf);
+    return wsum;
+}
+
+// FP32 sums over blocks of kViewBlock views, FP64 across blocks.
+constexpr int kViewBlock = 8;
+
+// Per-pixel accumulation over all views, every ray's tables and S sinogram values read
+// from global memory.
 template <int S, bool kBorder, typename TI, typename TO>
 __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, int s0)
 {
@@ -60,20 +136,7 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
     // pixel centre in mm, in the FP's frame (dx on both axes, centre at texel r - 1/2)
     const float x = (fi + 0.5f - A.r) * A.dx;
     const float y = (fj + 0.5f - A.r) * A.dx;
-    float xlo_c = -kBig, xhi_c = kBig, ylo_c = -kBig, yhi_c = kBig;
-    float xlo_b = -1.0f, xhi_b = 1.0f, ylo_b = -1.0f, yhi_b = 1.0f;
-    float rho = 1.41421356f;
-    if (kBorder) {
-        // clamp window of the sample coordinate relative to this pixel (border plateaus)
-        // and support of T_i in (u - i), unbounded on the outer side of border pixels
-        if (i == 0) xlo_c = 0.0f, xlo_b = -kBig;
-        if (i == A.nX - 1) xhi_c = 0.0f, xhi_b = kBig;
-        if (j == 0) ylo_c = 0.0f, ylo_b = -kBig;
-        if (j == A.nY - 1) yhi_c = 0.0f, yhi_b = kBig;
-        const float ex = (i == A.nX - 1) ? fmaxf(1.0f, A.far_x) : 1.0f;
-        const float ey = (j == A.nY - 1) ? fmaxf(1.0f, A.far_y) : 1.0f;
-        rho = sqrtf(ex * ex + ey * ey);
-    }
+    const PixelWin pw = pixel_window<kBorder>(A, i, j);
 
     const TI *sino[S];
 #pragma unroll
@@ -81,24 +144,13 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
         sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
 
     double acc[S];
+    float vsum[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = 0.0;
+    for (int s = 0; s < S; ++s) acc[s] = 0.0, vsum[s] = 0.0f;
 
     for (int v = 0; v < A.nViews; ++v) {
-        const float2 cs = A.cs[v];
-        const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
-        const float t = fmaf(x, cs.y, -y * cs.x);
-        const float rU = __builtin_amdgcn_rcpf(U);
-        const float cf = fmaf(atanf(t * rU), A.c_scale, A.c_off);
-        // Rays within distance rho (texels) of the centre subtend at most asin(rho dx / L)
-        // from the pixel's fan angle; L >= U and asin(z) <= z (1 + z^2) for z <= 0.9.
-        const float z = rho * A.dx * rU;
-        const float hw = z < 0.9f ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs, 1e-3f) : 1e9f;
-        const int clo = (int)fmaxf(ceilf(cf - hw), 0.0f);
-        const int chi = (int)fminf(floorf(cf + hw), (float)(A.nCols - 1));
-        float vsum[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) vsum[s] = 0.0f;
+        int clo, chi;
+        candidate_rays(A, A.cs[v], x, y, pw.rho, clo, chi);
         const size_t rbase = (size_t)v * A.nCols;
         for (int c = clo; c <= chi; ++c) {
             // The S sinogram values are loaded first, unconditionally, so their latency
@@ -107,39 +159,179 @@ __device__ __forceinline__ void adjoint_pixel(const FpaArgs &A, int i, int j, in
             const int N = A.ray_n[rbase + c];
             const float4 g = A.ray_geo[rbase + c];
             const float2 inv = A.ray_inv[rbase + c];
+            // one 32-bit byte offset serves all S sinograms (a slice's sinogram is < 4 GiB)
+            const unsigned boff = (unsigned)(rbase + c) * (unsigned)sizeof(TI);
             float sv[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) sv[s] = load_f(sino[s], rbase + c);
-            const float X0 = g.x - fi, Y0 = g.y - fj;
-            const float a1 = (xlo_b - X0) * inv.x, a2 = (xhi_b - X0) * inv.x;
-            const float b1 = (ylo_b - Y0) * inv.y, b2 = (yhi_b - Y0) * inv.y;
-            const float lo = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), 0.0f);
-            const float hi = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), (float)(N - 1));
-            float wsum = 0.0f;
-            const float kend = floorf(hi);
-            for (float kf = ceilf(lo); kf <= kend; kf += 1.0f) {
-                const float X = fmaf(kf, g.z, X0);
-                const float Y = fmaf(kf, g.w, Y0);
-                if (kBorder) {
-                    const float wx = 1.0f - fabsf(fminf(fmaxf(X, xlo_c), xhi_c));
-                    const float wy = 1.0f - fabsf(fminf(fmaxf(Y, ylo_c), yhi_c));
-                    wsum = fmaf(fmaxf(wx, 0.0f), fmaxf(wy, 0.0f), wsum);
-                } else {
-                    // inside the support |X|, |Y| < 1 up to rounding at its two ends
-                    wsum = fmaf(1.0f - fabsf(X), 1.0f - fabsf(Y), wsum);
-                }
-            }
+            for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+            const float wsum = ray_weight<kBorder>(g, inv, N, fi, fj, pw);
 #pragma unroll
             for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sv[s], vsum[s]);
         }
+        if (v % kViewBlock == kViewBlock - 1 || v == A.nViews - 1) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s];
+            for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s], vsum[s] = 0.0f;
+        }
     }
 #pragma unroll
     for (int s = 0; s < S; ++s)
         if (s0 + s < A.nSlices)
             store_f(static_cast<TO *>(A.img), (size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i,
                     (float)(acc[s] * (double)A.scale));
+}
+
+// ---------------------------------------------------------------- staged variant
+// For S >= 4 slices the per-ray loads dominate: S sinogram values and the ray's tables,
+// gathered per pixel.  The staged variant copies, per batch of kStageVB views, the 64
+// sinogram columns around the tile's candidate rays into LDS once (coalesced, slices
+// interleaved per column, converted to FP32) together with the columns' ray tables; a
+// pixel then reads a ray's S values with S/4 16-byte LDS reads.  The window of a view
+// is derived from the four corner pixels of the tile (t/U, and so the fan angle, is
+// extremal at a corner), lane 4m + c evaluating corner c of view m; a candidate column
+// outside the window is read from global memory.
+constexpr int kStageCols = 64;  // staged columns per view: one per lane
+constexpr int kStageVB = 4;     // views per staging batch
+
+template <int S>
+struct AdjStage {
+    static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
+    float val[kStageVB][kStageCols][kPitch];
+    float4 geo[kStageVB][kStageCols];
+    float2 inv[kStageVB][kStageCols];
+    int n[kStageVB][kStageCols];
+};
+
+template <int S, bool kBorder, typename TI, typename TO>
+__device__ __forceinline__ void adjoint_staged_tile(const FpaArgs &A, AdjStage<S> &st, int tx, int ty, int s0)
+{
+    static_assert(S % 4 == 0, "a staged column is read four slices at a time");
+    constexpr int VB = kStageVB;
+    constexpr int SQ = S / 4;  // slices staged per wave
+    const int i = tx * kAdjTx + (int)threadIdx.x % kAdjTx;
+    const int j = ty * kAdjTy + (int)threadIdx.x / kAdjTx;
+    const bool valid = i < A.nX && j < A.nY;  // no early exit: every thread stages and waits
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const float fi = (float)i, fj = (float)j;
+    const float x = (fi + 0.5f - A.r) * A.dx;
+    const float y = (fj + 0.5f - A.r) * A.dx;
+    const PixelWin pw = pixel_window<kBorder>(A, i, j);
+    // corner of this lane for the window: lane 4m + c evaluates corner c
+    const int ci = min(tx * kAdjTx + ((lane & 1) ? kAdjTx - 1 : 0), A.nX - 1);
+    const int cj = min(ty * kAdjTy + ((lane & 2) ? kAdjTy - 1 : 0), A.nY - 1);
+    const float cx = ((float)ci + 0.5f - A.r) * A.dx;
+    const float cy = ((float)cj + 0.5f - A.r) * A.dx;
+    // this wave stages slices [s0 + wave SQ, s0 + (wave + 1) SQ)
+    const TI *sino_w[SQ];
+#pragma unroll
+    for (int q = 0; q < SQ; ++q)
+        sino_w[q] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + wave * SQ + q, A.nSlices - 1) * A.sino_stride;
+    const TI *sino[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+
+    double acc[S];
+    float vsum[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0, vsum[s] = 0.0f;
+
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
+        int c_lo[VB];
+        {
+            int wlo, whi;
+            candidate_rays(A, A.cs[min(v0 + (lane >> 2) % VB, A.nViews - 1)], cx, cy, 1.41421356f, wlo, whi);
+#pragma unroll
+            for (int sh = 1; sh <= 2; sh <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh));
+#pragma unroll
+            for (int m = 0; m < VB; ++m)
+                c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kStageCols, 0));
+        }
+        __syncthreads();  // the previous batch's reads are done
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            const int c = c_lo[m] + lane;
+            if (v0 + m >= A.nViews || c >= A.nCols) continue;
+            const size_t rb = (size_t)(v0 + m) * A.nCols + c;
+            const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+            float q4[SQ];
+#pragma unroll
+            for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(sino_w[q], boff);
+            float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
+            if constexpr (SQ % 4 == 0) {
+#pragma unroll
+                for (int q = 0; q < SQ; q += 4)
+                    *reinterpret_cast<float4 *>(dst + q) = make_float4(q4[q], q4[q + 1], q4[q + 2], q4[q + 3]);
+            } else if constexpr (SQ == 2) {
+                *reinterpret_cast<float2 *>(dst) = make_float2(q4[0], q4[1]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
+            }
+            if (wave == 0) st.geo[m][lane] = A.ray_geo[rb];
+            else if (wave == 1) st.inv[m][lane] = A.ray_inv[rb];
+            else if (wave == 2) st.n[m][lane] = A.ray_n[rb];
+        }
+        __syncthreads();
+        if (!valid) continue;
+#pragma unroll 1
+        for (int m = 0; m < VB; ++m) {
+            const int v = v0 + m;
+            if (v >= A.nViews) break;
+            int clo, chi;
+            candidate_rays(A, A.cs[v], x, y, pw.rho, clo, chi);
+            for (int c = clo; c <= chi; ++c) {
+                const int lc = c - c_lo[m];
+                int N;
+                float4 g;
+                float2 inv;
+                float sv[S];
+                if ((unsigned)lc < (unsigned)kStageCols) {
+                    N = st.n[m][lc];
+                    g = st.geo[m][lc];
+                    inv = st.inv[m][lc];
+#pragma unroll
+                    for (int s = 0; s < S; s += 4) {
+                        const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
+                        sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
+                    }
+                } else {
+                    const size_t rb = (size_t)v * A.nCols + c;
+                    N = A.ray_n[rb];
+                    g = A.ray_geo[rb];
+                    inv = A.ray_inv[rb];
+                    const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+                }
+                const float wsum = ray_weight<kBorder>(g, inv, N, fi, fj, pw);
+#pragma unroll
+                for (int s = 0; s < S; ++s) vsum[s] = fmaf(wsum, sv[s], vsum[s]);
+            }
+            if (v % kViewBlock == kViewBlock - 1 || v == A.nViews - 1) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) acc[s] += (double)vsum[s], vsum[s] = 0.0f;
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        if (s0 + s < A.nSlices)
+            store_f(static_cast<TO *>(A.img), (size_t)(s0 + s) * A.img_stride + (size_t)j * A.nX + i,
+                    (float)(acc[s] * (double)A.scale));
+}
+
+template <int S, typename TI, typename TO>
+__global__ __launch_bounds__(kAdjThreads) void fp_adjoint_staged_kernel(FpaArgs A)
+{
+    __shared__ AdjStage<S> st;
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    const bool border_tile = tx == 0 || ty == 0 || (tx + 1) * kAdjTx >= A.nX || (ty + 1) * kAdjTy >= A.nY;
+    if (border_tile)
+        adjoint_staged_tile<S, true, TI, TO>(A, st, tx, ty, blockIdx.z * S);
+    else
+        adjoint_staged_tile<S, false, TI, TO>(A, st, tx, ty, blockIdx.z * S);
 }
 
 // Pixel tile (tx, ty) of TX x (256/TX) pixels for slices [s0, s0+S).
@@ -209,20 +401,23 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
 
 }  // namespace
 
+template <int S, typename TI, typename TO>
+static void launch_fpa_k(dim3 grid, hipStream_t stream, const FpaArgs &A)
+{
+    if constexpr (S >= 4)
+        hipLaunchKernelGGL((fp_adjoint_staged_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
+    else
+        hipLaunchKernelGGL((fp_adjoint_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
+}
+
 template <int S>
 static fbct_status launch_fpa_t(fbct_dtype ti, fbct_dtype to, dim3 grid, hipStream_t stream, const FpaArgs &A)
 {
-    const dim3 block(kAdjThreads);
-    if (ti == FBCT_F32 && to == FBCT_F32)
-        hipLaunchKernelGGL((fp_adjoint_kernel<S, float, float>), grid, block, 0, stream, A);
-    else if (ti == FBCT_BF16 && to == FBCT_F32)
-        hipLaunchKernelGGL((fp_adjoint_kernel<S, bf16_t, float>), grid, block, 0, stream, A);
-    else if (ti == FBCT_F32 && to == FBCT_BF16)
-        hipLaunchKernelGGL((fp_adjoint_kernel<S, float, bf16_t>), grid, block, 0, stream, A);
-    else if (ti == FBCT_BF16 && to == FBCT_BF16)
-        hipLaunchKernelGGL((fp_adjoint_kernel<S, bf16_t, bf16_t>), grid, block, 0, stream, A);
-    else
-        return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp_adjoint: unknown storage type");
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fpa_k<S, float, float>(grid, stream, A);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fpa_k<S, bf16_t, float>(grid, stream, A);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fpa_k<S, float, bf16_t>(grid, stream, A);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fpa_k<S, bf16_t, bf16_t>(grid, stream, A);
+    else return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp_adjoint: unknown storage type");
     return FBCT_OK;
 }
 

@@ -191,10 +191,11 @@ def test_batch_equals_per_slice(small, n):
         assert rel_l2(ba[s], host(plan.fp_adjoint(dev(y[s:s + 1])))[0]) <= 1e-6
 
 
-@pytest.mark.parametrize("n", [4, 16, 19])
+@pytest.mark.parametrize("n", [4, 8, 12, 16, 19])
 def test_clinical_batch_equals_per_slice(clinical, n):
     # the clinical grid selects the shared-slice kernels: FP with 4-slice tiles, the
-    # adjoint with 16-, 8-, 4- or 2-slice groups; a ragged last group is padded
+    # adjoint with 16-, 8- or 4-slice groups staged through LDS (one slice: the
+    # unstaged kernel); a ragged last group is padded
     plan, o, img = clinical
     x = np.stack([np.roll(img[0], 23 * s, axis=1) for s in range(n)])
     y = np.random.default_rng(n).normal(size=(n, 984, 888)).astype(np.float32)
