@@ -29,8 +29,9 @@ METRIC = "GUPS for FP & BP at 512²×984v; batched fwd+adjoint ms/slice vs plain
 # probe (377 B/ns per CU over 256 CUs); AMD publishes no LDS figure for this part.
 LDS_PEAK_GBS = 96700.0
 HBM_PEAK_GBS = 8000.0
-# HBM traffic of the dominant kernel measured with rocprofv3 counters (FETCH_SIZE and
-# WRITE_SIZE passes, calibrated on a plain copy); used when the workload matches.
+# Per-kernel HBM traffic (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, calibrated on a plain
+# copy) and VALU issue rate (SQ counters) of the clinical 64-slice dispatches; used when the
+# workload matches.
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_clinical_64slices.json")
 
 
@@ -157,7 +158,9 @@ def main(argv=None):
     reps = max(3, args.steps // 2)
     log("bench: per-operator timings")
     sino = target.contiguous()
-    ms_fp_bf16 = time_op(lambda: plan.fp(xb), reps)  # the step's FP launch (dominant kernel)
+    rb = (plan.fp(xb) - targetb).to(torch.bfloat16)
+    ms_fp_bf16 = time_op(lambda: plan.fp(xb), reps)  # the step's two launches
+    ms_fpa_bf16 = time_op(lambda: plan.fp_adjoint(rb), reps)
     ms_fp = time_op(lambda: plan.fp(x), reps)
     ms_fpa = time_op(lambda: plan.fp_adjoint(sino), reps)
     ms_bp = time_op(lambda: plan.bp(sino), reps)
@@ -174,19 +177,35 @@ def main(argv=None):
 
     ms_plain = time_op(plain, max(2, reps // 2))
 
-    # dominant kernel: the FP ray march of the step (BF16 images, one launch per fp call);
-    # its taps are gathered from an FP32 LDS tile: 4 taps x 4 B per sample and slice
-    fp_bytes = 4 * 4 * samples * S
-    achieved = fp_bytes / (ms_fp_bf16 * 1e-3) / 1e9
-    hbm_bytes = (2 * nX * nY + 4 * nV * nC) * S  # compulsory: BF16 image in + FP32 sinogram out
-    traffic = None
+    # Rooflines of the step's two launches.  Algorithmic work of FP and of its adjoint: 4
+    # gathered taps of s = 2 B (BF16 storage) per sample and slice; bound: on-chip gathers
+    # (LDS, measured peak).  Compulsory HBM bytes: the BF16 input and the FP32 output.
+    tr_k = {}
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
         if tr.get("slices") == S and tr.get("workload") == "clinical":
-            traffic = tr["kernels"]["fp_kernel_bf16"]["hbm_bytes"]
+            tr_k = tr["kernels"]
     except (OSError, ValueError, KeyError):
-        traffic = None
+        tr_k = {}
+
+    def roofline(kernel, key, ms, hbm_bytes):
+        alg = 4 * 2 * samples * S
+        ach = alg / (ms * 1e-3) / 1e9
+        k = tr_k.get(key, {})
+        return {
+            "kernel": kernel, "bound": "lds", "achieved": round(ach, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / LDS_PEAK_GBS, 4), "traffic": k.get("hbm_bytes"), "launch_ms": round(ms, 4),
+            "algorithmic_bytes_per_launch": alg, "compulsory_hbm_bytes_per_launch": hbm_bytes,
+            "hbm_frac_compulsory": round(hbm_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+            "valu_instr_per_cu_cycle": k.get("valu_per_cu_cycle"),
+            "peak_note": "LDS ds_read_b128 bandwidth measured on MI355X (no published figure); "
+                         "valu_instr_per_cu_cycle = SQ_INSTS_VALU / (256 CUs x cycles) from the counter profile",
+        }
+
+    rl_adj = roofline("fp_adjoint_staged_kernel<16, bf16, float>", "fp_adjoint_kernel_bf16", ms_fpa_bf16,
+                      (2 * nV * nC + 4 * nX * nY) * S)
+    rl_fp = roofline("fp_kernel<4, 8, bf16, float>", "fp_kernel_bf16", ms_fp_bf16, (2 * nX * nY + 4 * nV * nC) * S)
     out = {
         "metric": METRIC,
         "value": round(gups, 3),
@@ -210,6 +229,7 @@ def main(argv=None):
         "fp_gups": round(S * U / (ms_fp * 1e-3) / 1e9, 3),
         "fp_bf16_gups": round(S * U / (ms_fp_bf16 * 1e-3) / 1e9, 3),
         "fp_adjoint_gups": round(S * U / (ms_fpa * 1e-3) / 1e9, 3),
+        "fp_adjoint_bf16_gups": round(S * U / (ms_fpa_bf16 * 1e-3) / 1e9, 3),
         "bp_gups": round(S * U / (ms_bp * 1e-3) / 1e9, 3),
         "fbp_gups": round(S * U / (ms_fbp * 1e-3) / 1e9, 3),
         "ms_per_slice_batched": round(ms_step / S, 4),
@@ -217,20 +237,8 @@ def main(argv=None):
         "batched_vs_plain": round(ms_plain / ms_step, 3),
         "batched_f32_storage_vs_plain": round(ms_plain / ms_step_f32, 3),
         "wall_s": round(wall, 3),
-        "roofline": {
-            "kernel": "fp_kernel<4, 8, bf16, float>",
-            "bound": "lds",
-            "achieved": round(achieved, 1),
-            "peak": LDS_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / LDS_PEAK_GBS, 4),
-            "traffic": traffic,
-            "launch_ms": round(ms_fp_bf16, 4),
-            "algorithmic_bytes_per_launch": fp_bytes,
-            "compulsory_hbm_bytes_per_launch": hbm_bytes,
-            "hbm_frac_compulsory": round(hbm_bytes / (ms_fp_bf16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
-            "peak_note": "LDS ds_read_b128 bandwidth measured on MI355X (no published figure)",
-        },
+        "roofline": rl_adj,
+        "roofline_fp": rl_fp,
     }
 
     if not args.no_cpu_baseline:
@@ -405,7 +413,13 @@ def fdk_main(args):
         "ms_filter": round(ms_filter, 3),
         "ms_backproject": round(ms - ms_filter, 3),
         "backproject_gups": round(U / ((ms - ms_filter) * 1e-3) / 1e9, 3),
-        "ms_backproject_plain_layout": round(ms_bp, 3),
+        "ms_backproject_alone": round(ms_bp, 3),
+        # dominant kernel: the backprojection; algorithmic work 4 taps x 4 B (FP32 filtered
+        # projections) per voxel-view update, read from the LDS-staged detector footprint
+        "roofline": {"kernel": "cone_bp_lds_kernel<flat, 2 views per batch, float>", "bound": "lds",
+                     "achieved": round(16.0 * U / (ms_bp * 1e-3) / 1e9, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(16.0 * U / (ms_bp * 1e-3) / 1e9 / LDS_PEAK_GBS, 4), "traffic": None,
+                     "launch_ms": round(ms_bp, 4), "algorithmic_bytes_per_launch": 16 * U},
         "filter_roofline": {"bound": "mfma", "achieved": round(flops / (ms_filter * 1e-3) / 1e12, 3),
                             "peak": 2500.0, "unit": "TFLOP/s",
                             "frac": round(flops / (ms_filter * 1e-3) / 1e12 / 2500.0, 5),
