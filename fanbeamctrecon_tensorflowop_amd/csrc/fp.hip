@@ -175,23 +175,33 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
         __syncthreads();  // the previous slab's reads of `tile` are done
 
         // Stage taps [alo, alo+kSlab] x [mlo, mhi] of the S slices with clamp-to-edge
-        // addressing.  Global loads run along the image row (i) in both orientations.
-        const int F = xmajor ? kPitch : M;  // extent of the fast (row-contiguous) index
-        const int total = F * (xmajor ? M : kPitch);
-        const float invF = 1.0f / (float)F;
-        for (int idx = tid; idx < total; idx += kFpThreads) {
-            const int sidx = (int)(((float)idx + 0.5f) * invF);
-            const int fidx = idx - sidx * F;
-            const int aa = xmajor ? fidx : sidx;
-            const int mm = xmajor ? sidx : fidx;
-            const int ga = min(max(alo + aa, 0), na - 1);
-            const int gm = min(max(mlo + mm, 0), nm - 1);
-            const size_t go = xmajor ? (size_t)gm * A.nX + ga : (size_t)ga * A.nX + gm;
+        // addressing.  Global loads run along the image row (i) in both orientations, at
+        // 32-bit byte offsets from the slices' bases; each thread keeps one fixed column
+        // (x-major) or minor lane (y-major), so the index split is done once.
+        auto stage = [&](int aa, int mm, unsigned go) {
             TapT val;
             float *vp = reinterpret_cast<float *>(&val);
 #pragma unroll
-            for (int s = 0; s < S; ++s) vp[s] = load_f(img[s], go);
+            for (int s = 0; s < S; ++s) vp[s] = load_f_at(img[s], go * (unsigned)sizeof(TI));
             reinterpret_cast<TapT *>(tile)[mm * kPitch + aa] = val;
+        };
+        if (xmajor) {  // a = i: kRowsPerPass rows of kPitch taps per pass of the workgroup
+            constexpr int kRowsPerPass = kFpThreads / kPitch;
+            if (tid < kRowsPerPass * kPitch) {
+                const int aa = tid % kPitch;
+                const unsigned ga = (unsigned)min(max(alo + aa, 0), na - 1);
+                for (int mm = tid / kPitch; mm < M; mm += kRowsPerPass)
+                    stage(aa, mm, (unsigned)min(max(mlo + mm, 0), nm - 1) * (unsigned)A.nX + ga);
+            }
+        } else {  // a = j: 64 minor taps per wave, one major row per wave and pass
+            const int wave = tid >> 6;
+            for (int m0 = 0; m0 < M; m0 += 64) {
+                const int mm = m0 + (tid & 63);
+                if (mm >= M) continue;
+                const unsigned gm = (unsigned)min(max(mlo + mm, 0), nm - 1);
+                for (int aa = wave; aa < kPitch; aa += kFpThreads / 64)
+                    stage(aa, mm, (unsigned)min(max(alo + aa, 0), na - 1) * (unsigned)A.nX + gm);
+            }
         }
         __syncthreads();  // tile ready
 
