@@ -79,6 +79,8 @@ class ConeGeom(ctypes.Structure):
 
 
 # (name, restype, argtypes) of every symbol the header declares.
+FBP_COSINE, FBP_SHORT_SCAN = 1, 2  # fbct_fbp_ex flags
+
 _P = ctypes.c_void_p
 _F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.c_int
@@ -110,6 +112,7 @@ SIGNATURES = [
     ("fbct_ramp_filter_ex", _I, [_P, _P, _P, ctypes.c_int64, _I, _P]),
     ("fbct_fbp_workspace_size", ctypes.c_size_t, [_P, _I]),
     ("fbct_fbp", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
+    ("fbct_fbp_ex", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, ctypes.c_uint, _P]),
     ("fbct_fbp_adjoint", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fdk", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_batch_create", _I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),

@@ -158,21 +158,45 @@ size_t fbct_fbp_workspace_size(const fbct_plan *plan, int n_slices)
     return sino_bytes(plan, n_slices);
 }
 
-fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
-                     size_t workspace_bytes, void *stream)
+fbct_status fbct_fbp_ex(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
+                        size_t workspace_bytes, unsigned flags, void *stream)
 {
-    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp");
+    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp_ex");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if (flags & ~(unsigned)(FBCT_FBP_COSINE | FBCT_FBP_SHORT_SCAN))
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fbp_ex: unknown flags 0x%x", flags);
     if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
-        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp: workspace of %zu bytes needed, got %zu",
+        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_ex: workspace of %zu bytes needed, got %zu",
                     sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float *q = static_cast<float *>(workspace);
-    if ((s = filter_rows(*plan, *t, sino, q, (int64_t)n_slices * plan->g.nViews, FBCT_FILTER_MFMA, st)) != FBCT_OK)
-        return s;
+    const int64_t rows = (int64_t)n_slices * plan->g.nViews;
+    if (flags == 0) {
+        if ((s = filter_rows(*plan, *t, sino, q, rows, FBCT_FILTER_MFMA, st)) != FBCT_OK) return s;
+    } else {
+        // pre-weights multiply the rows as the MFMA filter stages them (row r = slice * nViews + view)
+        fbct::RowWeights rw;
+        if (flags & FBCT_FBP_COSINE) {
+            if ((s = fbct::device_cosine(plan, t, &rw.w)) != FBCT_OK) return s;
+            rw.wrows = 1;
+        }
+        if (flags & FBCT_FBP_SHORT_SCAN) {
+            if ((s = fbct::device_parker(plan, t, &rw.w2)) != FBCT_OK) return s;
+            rw.w2div = 1;
+            rw.w2rows = plan->g.nViews;
+        }
+        fbct::RampOperand H{t->ramp_planes, plan->g.nCols, plan->d.ramp_np, plan->d.ramp_kp};
+        if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, rows, rw, st)) != FBCT_OK) return s;
+    }
     return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);
+}
+
+fbct_status fbct_fbp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    return fbct_fbp_ex(plan, sino, image, n_slices, workspace, workspace_bytes, 0u, stream);
 }
 
 fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices, void *workspace,
@@ -195,25 +219,7 @@ fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, i
 fbct_status fbct_fbp_short_scan(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
                                 size_t workspace_bytes, void *stream)
 {
-    fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp_short_scan");
-    if (s != FBCT_OK || n_slices == 0) return s;
-    if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
-        return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_short_scan: workspace of %zu bytes needed, got %zu",
-                    sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
-    const fbct::DeviceTables *t = nullptr;
-    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
-    const float *pk = nullptr;
-    if ((s = fbct::device_parker(plan, t, &pk)) != FBCT_OK) return s;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    float *q = static_cast<float *>(workspace);
-    fbct::RampOperand H{t->ramp_planes, plan->g.nCols, plan->d.ramp_np, plan->d.ramp_kp};
-    fbct::RowWeights rw;
-    rw.w = pk;  // row r = slice * nViews + view
-    rw.wrows = plan->g.nViews;
-    if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, (int64_t)n_slices * plan->g.nViews, rw, st)) !=
-        FBCT_OK)
-        return s;
-    return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);
+    return fbct_fbp_ex(plan, sino, image, n_slices, workspace, workspace_bytes, FBCT_FBP_SHORT_SCAN, stream);
 }
 
 fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slices, void *workspace,

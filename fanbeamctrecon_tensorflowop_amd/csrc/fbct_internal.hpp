@@ -97,6 +97,7 @@ struct DeviceTables {
     double *ramp = nullptr;        // [2*nCols-1]
     uint16_t *ramp_planes = nullptr;  // [3][Np][Kp] BF16 split planes of the Toeplitz matrix H
     float *parker = nullptr;       // [nViews][nCols] short-scan weights (uploaded on first use)
+    float *cosine = nullptr;       // [nCols] FBP cosine pre-weights cos(gamma_c) (uploaded on first use)
 };
 
 // Scalars the kernels need, derived once in FP64 on the host.
@@ -250,6 +251,8 @@ bool parker_weights(const std::vector<float> &angles, const std::vector<double> 
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
 // The plan's Parker weights on the current device (computed and uploaded on first use).
 fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **out);
+// cos(gamma_c) per detector column (the backprojection's column map), uploaded on first use.
+fbct_status device_cosine(fbct_plan *plan, const DeviceTables *t, const float **out);
 
 // Kernel launchers (fp.hip, bp.hip, filter.hip).  All asynchronous on `stream`.
 fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti,

@@ -500,6 +500,7 @@ static void free_tables(DeviceTables &t)
     (void)hipFree(t.ramp);
     (void)hipFree(t.ramp_planes);
     (void)hipFree(t.parker);
+    (void)hipFree(t.cosine);
     if (cur >= 0) (void)hipSetDevice(cur);
     t = DeviceTables{};
 }
@@ -522,6 +523,26 @@ fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **
         if (s != FBCT_OK) return s;
     }
     *out = dt->parker;
+    return FBCT_OK;
+}
+
+fbct_status device_cosine(fbct_plan *plan, const DeviceTables *t, const float **out)
+{
+    std::lock_guard<std::mutex> lk(plan->mu);
+    DeviceTables *dt = nullptr;
+    for (DeviceTables &d : plan->dev)
+        if (&d == t) dt = &d;
+    if (!dt) return fail(FBCT_ERR_DEVICE, "cosine weights: tables not resident on this device");
+    if (!dt->cosine) {
+        std::vector<double> gamma;
+        double delta;
+        fan_gammas(plan->g, gamma, delta);
+        std::vector<float> w(gamma.size());
+        for (size_t c = 0; c < gamma.size(); ++c) w[c] = static_cast<float>(std::cos(gamma[c]));
+        fbct_status s = upload_vec(&dt->cosine, w);
+        if (s != FBCT_OK) return s;
+    }
+    *out = dt->cosine;
     return FBCT_OK;
 }
 

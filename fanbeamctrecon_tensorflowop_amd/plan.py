@@ -196,8 +196,28 @@ class FanBeamPlan:
         return self._run("fbct_fbp_backproject_adjoint",
                          self._prep(image, self.image_shape, "fbp_weighted_bp_adjoint"), self.sino_shape)
 
-    def fbp(self, sino: torch.Tensor) -> torch.Tensor:
-        return self._run("fbct_fbp", self._prep(sino, self.sino_shape, "fbp"), self.image_shape, extra_ws=True)
+    def fbp(self, sino: torch.Tensor, cosine: bool = False, short_scan: bool = False) -> torch.Tensor:
+        """FBP of sinograms [n, nViews, nCols] -> images [n, nY, nX].  The defaults are the
+        reference FBP op; cosine=True adds the equiangular cos(gamma) pre-weight the
+        reference omits, short_scan=True Parker's weights (fbct_fbp_ex flags)."""
+        x = self._prep(sino, self.sino_shape, "fbp")
+        flags = (_lib.FBP_COSINE if cosine else 0) | (_lib.FBP_SHORT_SCAN if short_scan else 0)
+        if flags == 0:
+            return self._run("fbct_fbp", x, self.image_shape, extra_ws=True)
+        lib = _lib.load()
+        n = x.shape[0]
+        out = torch.empty((n,) + tuple(self.image_shape), dtype=torch.float32, device=x.device)
+        if n == 0:
+            return out
+        with torch.cuda.device(x.device):
+            stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+            ws_bytes = int(lib.fbct_fbp_workspace_size(self._h, n))
+            ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=x.device)
+            st = lib.fbct_fbp_ex(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()), int(n),
+                                 ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws_bytes), ctypes.c_uint(flags),
+                                 stream)
+        _lib.check(st, "fbct_fbp_ex")
+        return out
 
     def fbp_adjoint(self, image: torch.Tensor) -> torch.Tensor:
         return self._run("fbct_fbp_adjoint", self._prep(image, self.image_shape, "fbp_adjoint"),

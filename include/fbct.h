@@ -204,6 +204,17 @@ fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, fl
 fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
                              void *workspace, size_t workspace_bytes, void *stream);
 
+/* FBP with optional pre-weights applied to the sinogram before the ramp filter (they
+ * multiply the rows as the MFMA filter stages them; one launch):
+ *   FBCT_FBP_COSINE: cos(gamma_c) per column, gamma_c = dLeft + (c + 1/2 - dShift) dCol,
+ *     the equiangular FBP's cosine weighting, which the reference omits
+ *     (FBP/transform.cu:162); the reference's FBP is flags = 0;
+ *   FBCT_FBP_SHORT_SCAN: Parker's weights (as fbct_fbp_short_scan).
+ * Flags combine.  No reference counterpart beyond flags = 0 (= fbct_fbp). */
+enum { FBCT_FBP_COSINE = 1, FBCT_FBP_SHORT_SCAN = 2 };
+fbct_status fbct_fbp_ex(fbct_plan *plan, const float *sino, float *image, int n_slices, void *workspace,
+                        size_t workspace_bytes, unsigned flags, void *stream);
+
 /* Short-scan FBP: the sinogram is multiplied by fbct_plan_parker_weights before the
  * ramp filter (one MFMA launch), then backprojected as fbct_fbp.  For a scan over
  * 180 degrees plus the fan angle (parameter[3] gives the angular range).  The first call

@@ -743,3 +743,33 @@ int orc_fbp_short_scan(const orc_geom *g, const float *angles, const double *sin
     free(w);
     return 0;
 }
+
+/* FBP with pre-weights on the sinogram before the filter: cos(gamma_c) (the equiangular
+ * FBP's cosine weighting, Hsieh eq. 3.51; the reference's FBP omits it,
+ * FBP/transform.cu:162) and/or Parker's weights.  gamma_c is the column's fan angle as
+ * the backprojection maps it, dLeft + (c + 1/2 - dShift) dCol. */
+int orc_fbp_weighted(const orc_geom *g, const float *angles, const double *sino, double *img, int nS, int cosine,
+                     int short_scan, int nthreads)
+{
+    const long long per = (long long)g->nViews * g->nCols;
+    double *w = (double *)malloc(sizeof(double) * per);
+    if (short_scan) {
+        if (orc_parker_fan(g, angles, w) != 0) {
+            free(w);
+            return -1;
+        }
+    } else {
+        for (long long k = 0; k < per; ++k) w[k] = 1.0;
+    }
+    if (cosine)
+        for (long long k = 0; k < per; ++k) {
+            const int c = (int)(k % g->nCols);
+            w[k] *= cos((double)g->dLeft + ((double)c + 0.5 - (double)g->dShift) * (double)g->dCol);
+        }
+    double *p = (double *)malloc(sizeof(double) * (nS * per > 0 ? nS * per : 1));
+    for (long long k = 0; k < nS * per; ++k) p[k] = sino[k] * w[k % per];
+    orc_fbp(g, angles, p, img, nS, nthreads);
+    free(p);
+    free(w);
+    return 0;
+}

@@ -185,6 +185,12 @@ int orc_parker_fan(const orc_geom *g, const float *angles, double *w);
 int orc_fbp_short_scan(const orc_geom *g, const float *angles, const double *sino, double *img, int nS,
                        int nthreads);
 
+/* FBP with the cosine pre-weight cos(gamma_c), gamma_c = dLeft + (c + 1/2 - dShift) dCol
+ * (cosine != 0), and/or Parker's weights (short_scan != 0) on the sinogram before the
+ * filter.  Returns -1 if short_scan is asked for and the scan is too short. */
+int orc_fbp_weighted(const orc_geom *g, const float *angles, const double *sino, double *img, int nS, int cosine,
+                     int short_scan, int nthreads);
+
 /* Threads the oracle will use when nthreads <= 0. */
 int orc_default_threads(void);
 

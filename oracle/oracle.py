@@ -100,6 +100,8 @@ def _load():
     lib.orc_parker_fan.restype = i
     lib.orc_fbp_short_scan.argtypes = [g, f, d, d, i, i]
     lib.orc_fbp_short_scan.restype = i
+    lib.orc_fbp_weighted.argtypes = [g, f, d, d, i, i, i, i]
+    lib.orc_fbp_weighted.restype = i
     lib.orc_default_threads.argtypes = []
     lib.orc_default_threads.restype = i
     _lib = lib
@@ -228,6 +230,15 @@ class Oracle:
         out = np.zeros((p.shape[0],) + self.img_shape, dtype=np.float64)
         if _load().orc_fbp_short_scan(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0],
                                       nthreads) != 0:
+            raise ValueError("short scan: the views do not cover pi + the fan angle")
+        return out
+
+    def fbp_weighted(self, sino, cosine=False, short_scan=False, nthreads=0):
+        """FBP with the cosine pre-weight and/or Parker's weights (orc_fbp_weighted)."""
+        p = self._sino(sino)
+        out = np.zeros((p.shape[0],) + self.img_shape, dtype=np.float64)
+        if _load().orc_fbp_weighted(ctypes.byref(self.g), _fptr(self.angles), _dptr(p), _dptr(out), p.shape[0],
+                                    int(bool(cosine)), int(bool(short_scan)), nthreads) != 0:
             raise ValueError("short scan: the views do not cover pi + the fan angle")
         return out
 

@@ -327,3 +327,28 @@ def test_fbp_reconstructs_disk_roughly(oracle_mod):
     centre = rec[28:36, 28:36].mean()
     assert centre == pytest.approx(0.02, rel=0.15)
     assert abs(rec[2:6, 2:6].mean()) < 0.1 * 0.02
+
+
+@pytest.mark.parametrize("dshift", [0.0, 0.5])
+def test_fbp_cosine_weight_reconstructs_flat_disk(oracle_mod, dshift):
+    """The equiangular FBP with the cos(gamma) pre-weight (Hsieh eq. 3.51) reconstructs a
+    centred uniform disk from its exact fan-beam chords flat to 0.5% over the disk on a 60
+    degree fan; without the weight (the reference FBP) the centre is low and the rim high
+    by about 1% each."""
+    nc = 256
+    p, a = workloads.scan(nx=96, rfov=48.0, sid=150.0, sdd=300.0, n_cols=nc, dcol_rad=math.radians(60.0) / nc,
+                          n_views=360, dshift=dshift)
+    o = oracle_mod.Oracle(p, a)
+    R, rho, mu = float(o.g.R), 30.0, 0.02
+    gam = float(o.g.dLeft) + (np.arange(nc) + 0.5 - float(o.g.dShift)) * float(o.g.dCol)
+    row = 2.0 * mu * np.sqrt(np.clip(rho ** 2 - (R * np.sin(gam)) ** 2, 0.0, None))  # chord at distance R sin g
+    sino = np.tile(row, (len(a), 1))[None]
+    x = -48.0 + (np.arange(96) + 0.5)
+    r = np.hypot(*np.meshgrid(x, x))
+    rings = [r < 5.0, (r > 15.0) & (r < 20.0), (r > 24.0) & (r < 27.0)]
+    weighted = o.fbp_weighted(sino, cosine=True)[0]
+    plain = o.fbp_weighted(sino, cosine=False)[0]
+    np.testing.assert_allclose(plain, o.fbp(sino)[0], rtol=1e-12, atol=1e-15)
+    for m in rings:
+        assert weighted[m].mean() == pytest.approx(mu, rel=5e-3)
+    assert plain[rings[2]].mean() / plain[rings[0]].mean() > 1.015
