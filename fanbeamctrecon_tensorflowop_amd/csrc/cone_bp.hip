@@ -12,10 +12,10 @@
 // bilinear blend.  Texture semantics are reproduced with exact FP32 weights.  Near the
 // two fan edges the in-range decision is retaken in FP64 on t/U.
 //
-// Volumes of 16 slices or more use cone_bp_lds_kernel, which stages each workgroup's
-// detector footprint in LDS (the taps of a 16 x 16 x 16 voxel tile span about 30 x 25
-// detector samples per view, read by 4096 voxels); thinner volumes read the taps from
-// global memory (cone_bp_kernel).
+// Volumes of 16 slices or more use cone_bp_lds_kernel (z-chunks of 32 voxels from 32
+// slices on), which stages each workgroup's detector footprint in LDS (the taps of a
+// 16 x 16 x 16 voxel tile span about 30 x 25 detector samples per view, read by 4096
+// voxels); thinner volumes read the taps from global memory (cone_bp_kernel).
 
 #include <hip/hip_runtime.h>
 
@@ -147,21 +147,19 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
         if (k0 + k < A.nZ) store_f(out, (size_t)(k0 + k) * A.nY * A.nX, acc[k]);
 }
 
-// LDS-staged variant for z-chunks of kLdsKZ voxels (plain layout).  The workgroup's
+// LDS-staged variant for z-chunks of KZ = 16 or 32 voxels (plain layout).  The workgroup's
 // detector footprint per view is derived from the four corner columns of its 16 x 16
 // tile (t/U, and so the column coordinate, is extremal at a corner; the row coordinate at
 // a corner and an end of the z-chunk), kLdsGroup views at a time: lane 4m + c evaluates
 // corner c of view m.  Per batch of VB views the workgroup stages each footprint's rows
 // once (one detector row per wave instruction, lane = column, indices clamped to the
 // detector, which reproduces clamp-to-edge), and every voxel reads its four taps from
-// LDS.  A column whose taps leave the staged window (a footprint taller than kLdsRows or
-// wider than kLdsCols) reads them from global memory instead.
-constexpr int kLdsKZ = 16;
-constexpr int kLdsVB = 2;                      // views per staging batch (2: 23 KB of LDS)
+// LDS.  A column whose taps leave the staged window (a footprint taller than
+// lds_rows<KZ>() or wider than kLdsCols) reads them from global memory instead.
+constexpr int kLdsVB = 2;                      // views per staging batch
 constexpr int kLdsGroup = 16;                  // views per footprint evaluation (64 lanes / 4 corners)
 constexpr int kLdsCols = 64;                   // staged columns per row: one per lane
 constexpr int kLdsPitch = kLdsCols + 1;        // floats between the views of a staged row
-constexpr int kLdsRows = 44;                   // staged rows per view
 // Staged element (view m, row r, column c) sits at r * stride + m * kLdsPitch + c: the
 // four taps of a voxel of any view of the batch are two ds_read2 with immediate offsets
 // (at most 255 dwords), and the odd row stride keeps adjacent rows off the same banks.
@@ -169,6 +167,14 @@ template <int VB>
 constexpr int lds_stride()
 {
     return (VB * kLdsPitch) | 1;
+}
+// Staged rows per view for z-chunks of KZ voxels: a 16 x 16 x KZ tile's taps span at most
+// 42 (KZ = 16) or 71 (KZ = 32) detector rows at the 0.7 mm / 1.0 mm, SID 541, SDD 949
+// geometry (magnification up to 2.6 at the FOV edge), margin included.
+template <int KZ>
+constexpr int lds_rows()
+{
+    return KZ == 32 ? 72 : 44;
 }
 
 __device__ __forceinline__ int floor_to_int(float v)
@@ -185,10 +191,10 @@ __device__ __forceinline__ int cvt_floor(float v)
     return r;
 }
 
-template <bool kFlat, int kNc, int VB, typename TO>
+template <bool kFlat, int kNc, int KZ, int VB, typename TO>
 __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
 {
-    constexpr int KZ = kLdsKZ;
+    constexpr int kLdsRows = lds_rows<KZ>();
     constexpr int RS = lds_stride<VB>();
     static_assert(kLdsGroup % VB == 0, "a batch of views lies inside one footprint group");
     static_assert(RS + (VB - 1) * kLdsPitch + 1 <= 255, "tap offsets fit ds_read2");
@@ -353,14 +359,14 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
         if (k0 + k < A.nZ) store_f(out, (size_t)(k0 + k) * A.nY * A.nX, acc[k]);
 }
 
-template <bool kFlat, int kNc>
+template <bool kFlat, int kNc, int KZ>
 void launch_lds(dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
     constexpr int VB = kLdsVB;
     if (to == FBCT_BF16)
-        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, VB, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, KZ, VB, bf16_t>), grid, dim3(kTx * kTy), 0, st, A);
     else
-        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, VB, float>), grid, dim3(kTx * kTy), 0, st, A);
+        hipLaunchKernelGGL((cone_bp_lds_kernel<kFlat, kNc, KZ, VB, float>), grid, dim3(kTx * kTy), 0, st, A);
 }
 
 template <bool kFlat, int kNc, int KZ>
@@ -375,7 +381,8 @@ void launch_kz(dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 template <bool kFlat, int kNc>
 void launch_nc(int kz, dim3 grid, hipStream_t st, const ConeArgs &A, fbct_dtype to)
 {
-    if (kz == kLdsKZ) launch_lds<kFlat, kNc>(grid, st, A, to);
+    if (kz == 32) launch_lds<kFlat, kNc, 32>(grid, st, A, to);
+    else if (kz == 16) launch_lds<kFlat, kNc, 16>(grid, st, A, to);
     else if (kz == 4) launch_kz<kFlat, kNc, 4>(grid, st, A, to);
     else launch_kz<kFlat, kNc, 1>(grid, st, A, to);
 }
@@ -412,7 +419,7 @@ fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, c
     A.tan_min = p.d.tan_min;
     A.tan_max = p.d.tan_max;
     for (int k = 0; k < kBpAtanMax; ++k) A.atan_c[k] = p.d.atan_c[k];
-    const int kz = g.nZ >= 16 ? 16 : (g.nZ >= 4 ? 4 : 1);
+    const int kz = g.nZ >= 32 ? 32 : (g.nZ >= 16 ? 16 : (g.nZ >= 4 ? 4 : 1));
     A.nzc = (g.nZ + kz - 1) / kz;
     const long long gz = (long long)A.nzc * n;
     if (gz > 65535) return fail(FBCT_ERR_UNSUPPORTED, "cone backprojection: %lld z-groups exceed the grid", gz);
