@@ -144,6 +144,122 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
     for (int s = 0; s < S; ++s) A.img[(sl + s) * A.img_stride + (size_t)j * A.nX + i] = acc[s];
 }
 
+// Staged variant for S = 4, 8, 16 slices: per batch of kBpStageVB views the workgroup
+// copies the 64 detector columns around its tile into LDS once (coalesced, the S slices
+// of a column contiguous, so a tap of all S slices is S/4 16-byte LDS reads instead of
+// S gathers).  The window of a view starts at the smallest column coordinate of the
+// tile's four corner pixels (t/U, and so xi, is extremal at a corner; lane 4m + c
+// evaluates corner c of view m); a pixel whose taps leave the window reads them from
+// global memory.  Results are those of bp_kernel (same arithmetic per update).
+constexpr int kBpStageCols = 64;
+constexpr int kBpStageVB = 4;
+
+template <int S>
+struct BpStage {
+    static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
+    float val[kBpStageVB][kBpStageCols][kPitch];
+};
+
+template <bool kWeighted, int S, int kNc>
+__global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int slice0)
+{
+    static_assert(S % 4 == 0, "a staged column is read four slices at a time");
+    constexpr int VB = kBpStageVB, SQ = S / 4;
+    __shared__ BpStage<S> st;
+    unsigned t, g;  // XCD-aware (tile, slice group)
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
+    const int tx0 = (int)(t % gridDim.x) * kBpTx, ty0 = (int)(t / gridDim.x) * kBpTy;
+    const int i = tx0 + (int)threadIdx.x % kBpTx;
+    const int j = ty0 + (int)threadIdx.x / kBpTx;
+    const size_t sl = (size_t)slice0 + (size_t)S * g;
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    // no early exit: every thread stages and waits
+    const bool valid = i < A.nX && j < A.nY;
+    const PixelGeo pg = pixel_geo(A, min(i, A.nX - 1), min(j, A.nY - 1));
+    const bool inside = valid && pg.x * pg.x + pg.y * pg.y <= A.r2;
+    const PixelGeo pc = pixel_geo(A, min(tx0 + ((lane & 1) ? kBpTx - 1 : 0), A.nX - 1),
+                                  min(ty0 + ((lane & 2) ? kBpTy - 1 : 0), A.nY - 1));
+    const float *sino = A.sino + sl * A.sino_stride;  // slice group base
+    const float *sino_w = sino + (size_t)(wave * SQ) * A.sino_stride;  // this wave stages slices wave*SQ..
+    float acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
+        int c_lo[VB];
+        {
+            float xi, U, tt;
+            (void)column_of<kNc>(A, pc, min(v0 + (lane >> 2) % VB, A.nViews - 1), xi, U, tt);
+            int wlo = (int)floorf(fminf(fmaxf(xi - 0.5f, -1.0e6f), 1.0e6f));
+#pragma unroll
+            for (int sh = 1; sh <= 2; sh <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh));
+#pragma unroll
+            for (int m = 0; m < VB; ++m)
+                c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kBpStageCols, 0));
+        }
+        __syncthreads();  // the previous batch's reads are done
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            const int c = c_lo[m] + lane;
+            if (v0 + m >= A.nViews || c >= A.nCols) continue;
+            const float *src = sino_w + (size_t)(v0 + m) * A.nCols + c;
+            float q[SQ];
+#pragma unroll
+            for (int k = 0; k < SQ; ++k) q[k] = src[(size_t)k * A.sino_stride];
+            float *dst = &st.val[m][lane][wave * SQ];
+            if constexpr (SQ % 4 == 0) {
+#pragma unroll
+                for (int k = 0; k < SQ; k += 4)
+                    *reinterpret_cast<float4 *>(dst + k) = make_float4(q[k], q[k + 1], q[k + 2], q[k + 3]);
+            } else if constexpr (SQ == 2) {
+                *reinterpret_cast<float2 *>(dst) = make_float2(q[0], q[1]);
+            } else {
+                dst[0] = q[0];
+            }
+        }
+        __syncthreads();
+        if (!inside) continue;
+#pragma unroll 1
+        for (int m = 0; m < VB; ++m) {
+            const int v = v0 + m;
+            if (v >= A.nViews) break;
+            float xi, U, tt;
+            if (!column_of<kNc>(A, pg, v, xi, U, tt)) continue;
+            float al;
+            const int c0 = interp_pos(xi, A.nCols, al);
+            const float w = kWeighted ? __builtin_amdgcn_rcpf(fmaf(U, U, tt * tt)) : 1.0f;
+            const int lc = c0 - c_lo[m];
+            float p0[S], p1[S];
+            if ((unsigned)lc < (unsigned)(kBpStageCols - 1) && c0 + 1 < A.nCols) {
+#pragma unroll
+                for (int s = 0; s < S; s += 4) {
+                    const float4 a = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
+                    const float4 b = *reinterpret_cast<const float4 *>(&st.val[m][lc + 1][s]);
+                    p0[s] = a.x, p0[s + 1] = a.y, p0[s + 2] = a.z, p0[s + 3] = a.w;
+                    p1[s] = b.x, p1[s + 1] = b.y, p1[s + 2] = b.z, p1[s + 3] = b.w;
+                }
+            } else {
+                const float *P = sino + (size_t)v * A.nCols;
+                const int c1 = min(c0 + 1, A.nCols - 1);
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    p0[s] = P[(size_t)s * A.sino_stride + c0];
+                    p1[s] = P[(size_t)s * A.sino_stride + c1];
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const float val = fmaf(al, p1[s] - p0[s], p0[s]);
+                acc[s] = kWeighted ? fmaf(val, w, acc[s]) : acc[s] + val;
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int s = 0; s < S; ++s) A.img[(sl + s) * A.img_stride + (size_t)j * A.nX + i] = acc[s];
+}
+
 // Transpose: a workgroup owns one detector row (view v, slice) in LDS, visits every
 // pixel of the FOV and scatters the two linear-interpolation weights (clamped taps
 // folded into the edge columns) with ds_add_f32; the row is stored once.
@@ -207,17 +323,18 @@ BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, f
 template <bool kWeighted, int kNc>
 void launch_bp_groups(const BpArgs &A, dim3 tiles, int n, hipStream_t stream)
 {
-    // slice groups of 8/4/2 while they leave enough workgroups to fill the chip, the
-    // remainder one slice per group
+    // slice groups of 16/8/4/2 while they leave enough workgroups to fill the chip, the
+    // remainder one slice per group; groups of 4 or more stage their detector columns
     const long long nt = (long long)tiles.x * tiles.y;
     int done = 0;
-    for (int S : {8, 4, 2, 1}) {
+    for (int S : {16, 8, 4, 2, 1}) {
         const int groups = (n - done) / S;
         if (groups == 0 || (S > 1 && nt * groups < 2048)) continue;
         const dim3 grid(tiles.x, tiles.y, groups);
         switch (S) {
-        case 8: hipLaunchKernelGGL((bp_kernel<kWeighted, 8, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
-        case 4: hipLaunchKernelGGL((bp_kernel<kWeighted, 4, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        case 16: hipLaunchKernelGGL((bp_staged_kernel<kWeighted, 16, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        case 8: hipLaunchKernelGGL((bp_staged_kernel<kWeighted, 8, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
+        case 4: hipLaunchKernelGGL((bp_staged_kernel<kWeighted, 4, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
         case 2: hipLaunchKernelGGL((bp_kernel<kWeighted, 2, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
         default: hipLaunchKernelGGL((bp_kernel<kWeighted, 1, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
         }

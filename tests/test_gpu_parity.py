@@ -205,6 +205,18 @@ def test_clinical_batch_equals_per_slice(clinical, n):
         assert rel_l2(ba[s], host(plan.fp_adjoint(dev(y[s:s + 1])))[0]) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [8, 16, 32, 37])
+def test_clinical_bp_groups_equal_per_slice(clinical, n):
+    # on the clinical grid BP and FBP run slice groups of 16, 8 or 4 that stage their
+    # detector columns in LDS (37 = 2 x 16 + 4 + 1: a staged group and an unstaged slice)
+    plan, o, img = clinical
+    y = np.random.default_rng(n).normal(size=(n, 984, 888)).astype(np.float32)
+    bb, bf = host(plan.bp(dev(y))), host(plan.fbp(dev(y)))
+    for s in (0, n // 2, n - 1):
+        assert rel_l2(bb[s], host(plan.bp(dev(y[s:s + 1])))[0]) <= 1e-6
+        assert rel_l2(bf[s], host(plan.fbp(dev(y[s:s + 1])))[0]) <= 1e-6
+
+
 def test_zero_slices():
     plan = FanBeamPlan(*workloads.tiny_scan())
     out = plan.fp(torch.zeros(0, 16, 16, device="cuda"))
