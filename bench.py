@@ -416,7 +416,7 @@ def fdk_main(args):
         "ms_backproject_alone": round(ms_bp, 3),
         # dominant kernel: the backprojection; algorithmic work 4 taps x 4 B (FP32 filtered
         # projections) per voxel-view update, read from the LDS-staged detector footprint
-        "roofline": {"kernel": "cone_bp_lds_kernel<flat, 2 views per batch, float>", "bound": "lds",
+        "roofline": {"kernel": "cone_bp_lds_kernel<flat, 32-voxel z-chunks, 2 views per batch, float>", "bound": "lds",
                      "achieved": round(16.0 * U / (ms_bp * 1e-3) / 1e9, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
                      "frac": round(16.0 * U / (ms_bp * 1e-3) / 1e9 / LDS_PEAK_GBS, 4), "traffic": None,
                      "launch_ms": round(ms_bp, 4), "algorithmic_bytes_per_launch": 16 * U},
