@@ -166,9 +166,11 @@ fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch 
                 fp_items[k] += (int64_t)p.g.nViews * p.d.fpb[k].nb;
                 b->fp_cap_max[k] = std::max(b->fp_cap_max[k], p.d.fpb[k].cap);
             }
+            const bool cells = fbct::adj_cells_fit(p.d.fpa_far_x, p.d.fpa_far_y);
             for (int k = 0; k < fbct::kAdjNumTiles; ++k) {
                 const int tx = fbct::kAdjTileX[k], ty = 256 / tx;
-                fpa_items[k] += (int64_t)((p.g.nX + tx - 1) / tx) * ((p.g.nY + ty - 1) / ty);
+                fpa_items[k] += cells ? (int64_t)fbct::adj_cell_tiles(p.g.nX) * fbct::adj_cell_tiles(p.g.nY)
+                                      : (int64_t)((p.g.nX + tx - 1) / tx) * ((p.g.nY + ty - 1) / ty);
             }
             for (int64_t v : fp_items)
                 if (v > INT_MAX) {

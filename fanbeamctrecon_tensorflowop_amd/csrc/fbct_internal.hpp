@@ -59,6 +59,13 @@ struct FpBundleCfg {
 constexpr int kAdjNumTiles = 4;
 constexpr int kAdjTileX[kAdjNumTiles] = {16, 32, 8, 64};
 
+// FP^T cell tiles: kAdjCellTile^2 texel cells per workgroup yield (kAdjCellTile-1)^2
+// pixels.  The cell-driven kernels serve a plan whose samples all lie in the cells
+// [-1, n-1] of both axes (adj_cells_fit).
+constexpr int kAdjCellTile = 16;
+constexpr int adj_cell_tiles(int n) { return (n + kAdjCellTile - 2) / (kAdjCellTile - 1); }
+constexpr bool adj_cells_fit(float far_x, float far_y) { return far_x < 1.0f && far_y < 1.0f; }
+
 // Per-slice descriptors of an uneven batch (device resident).  Offsets count elements
 // of the storage type from the packed buffers' bases.
 struct FpSliceDesc {

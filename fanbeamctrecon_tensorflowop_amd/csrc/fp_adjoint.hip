@@ -355,13 +355,139 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_kernel(FpaArgs A)
     adjoint_tile<S, kAdjTx, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S);
 }
 
-// Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, pixel tile),
-// items of slice s are [prefix[s], prefix[s+1]).  A fixed grid strides over all items.
+// ---------------------------------------------------------------- cell-driven variant
+// For one or two slices the per-pixel gather evaluates each sample once for every pixel
+// of its 2x2 support.  Here a thread owns a texel cell (ci, cj), the square between the
+// pixel centres ci..ci+1 and cj..cj+1 (samples inside it interpolate between those four
+// pixels).  Per view and candidate ray (lines within half a cell diagonal of the cell
+// centre) it finds the ray's samples inside the cell: along each axis the samples before
+// boundary b number ceil((b - u0) / du) (du > 0; floor + 1 for du < 0), evaluated by the
+// same expression in both cells that share b, so the cells partition every ray's samples
+// exactly.  The four bilinear weight sums of those n samples (fractions ta + j du,
+// tm + j dw) are closed-form, as in the forward kernel's cell walk, and accumulate into
+// four corner partials.  Corner partials are combined per pixel through LDS at the end,
+// clamp-to-edge folding the outer corners of cells -1 and n-1 into the border pixels.
+// Valid while every sample lies in the cells [-1, n-1] of both axes (2 rFOV/dx - n + 1/2
+// < 1, square images); otherwise the gather kernels run.
+constexpr int kCellTile = kAdjCellTile;  // cells per tile side; a tile yields 15 x 15 pixels
+
+// Cell tile (tx, ty): cells tx*15-1 .. tx*15+14 (and likewise in y), output pixels
+// tx*15 .. tx*15+14.  Every thread of the workgroup calls it (two barriers inside);
+// part holds 4 * S * 256 doubles of LDS.
+template <int S, typename TI, typename TO>
+__device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int s0, double *part)
+{
+    constexpr int kOut = kCellTile - 1;
+    constexpr int kCells = kCellTile * kCellTile;
+    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    const int ci = tx * kOut - 1 + lx, cj = ty * kOut - 1 + ly;  // >= -1
+    const float fci = (float)ci, fcj = (float)cj;
+    // cell centre in mm: texel coordinate ci + 1/2 (texel u sits at (u + 1/2 - r) dx)
+    const float x = (fci + 1.0f - A.r) * A.dx;
+    const float y = (fcj + 1.0f - A.r) * A.dx;
+    const TI *sino[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+    double acc[4][S];
+    float blk[4][S];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[q][s] = 0.0, blk[q][s] = 0.0f;
+
+    if (ci <= A.nX - 1 && cj <= A.nY - 1) {
+        for (int v = 0; v < A.nViews; ++v) {
+            int clo, chi;
+            candidate_rays(A, A.cs[v], x, y, 0.70710678f, clo, chi);
+            const size_t rbase = (size_t)v * A.nCols;
+            for (int c = clo; c <= chi; ++c) {
+                const int N = A.ray_n[rbase + c];
+                const float4 g = A.ray_geo[rbase + c];
+                const float2 inv = A.ray_inv[rbase + c];
+                const unsigned boff = (unsigned)(rbase + c) * (unsigned)sizeof(TI);
+                float sv[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+                // sample range inside the cell, per axis, then intersected with [0, N)
+                const float qu0 = (fci - g.x) * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
+                const float qw0 = (fcj - g.y) * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
+                const bool pu = g.z >= 0.0f, pw = g.w >= 0.0f;
+                const float lou = pu ? ceilf(qu0) : floorf(qu1) + 1.0f, hiu = pu ? ceilf(qu1) : floorf(qu0) + 1.0f;
+                const float low = pw ? ceilf(qw0) : floorf(qw1) + 1.0f, hiw = pw ? ceilf(qw1) : floorf(qw0) + 1.0f;
+                const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), (float)N);
+                const float khi = fminf(fminf(hiu, hiw), (float)N);
+                const float n = fmaxf(khi - klo, 0.0f);
+                const float ta = fmaf(klo, g.z, g.x) - fci, tm = fmaf(klo, g.w, g.y) - fcj;
+                const float t1 = 0.5f * fmaf(n, n, -n);                    // sum of j < n
+                const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);  // sum of j^2
+                const float sa = fmaf(g.z, t1, n * ta);                      // sum of u-fractions
+                const float sm = fmaf(g.w, t1, n * tm);                      // sum of w-fractions
+                const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, g.w, tm * g.z), t1, g.z * g.w * t2));
+                const float w[4] = {(n - sa) - (sm - sam), sa - sam, sm - sam, sam};  // corners (0,0) (1,0) (0,1) (1,1)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) blk[q][s] = fmaf(w[q], sv[s], blk[q][s]);
+            }
+            if (v % kViewBlock == kViewBlock - 1 || v == A.nViews - 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) acc[q][s] += (double)blk[q][s], blk[q][s] = 0.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < S; ++s) part[(q * S + s) * kCells + threadIdx.x] = acc[q][s];
+    __syncthreads();
+    // pixel (px, py) collects corner dx of cell px - dx (dx = 0, 1), and, on the borders,
+    // the clamped outer corner of cell -1 (pixel 0) or of cell n-1 (pixel n-1)
+    const int px = tx * kOut + lx, py = ty * kOut + ly;
+    if (lx < kOut && ly < kOut && px < A.nX && py < A.nY) {
+        int xc[3], xd[3], yc[3], yd[3];  // local cell column and corner of each x (y) term
+        int nxt = 0, nyt = 0;
+        xc[nxt] = lx + 1, xd[nxt++] = 0;
+        xc[nxt] = lx, xd[nxt++] = 1;
+        if (px == 0) xc[nxt] = lx, xd[nxt++] = 0;
+        if (px == A.nX - 1) xc[nxt] = lx + 1, xd[nxt++] = 1;
+        yc[nyt] = ly + 1, yd[nyt++] = 0;
+        yc[nyt] = ly, yd[nyt++] = 1;
+        if (py == 0) yc[nyt] = ly, yd[nyt++] = 0;
+        if (py == A.nY - 1) yc[nyt] = ly + 1, yd[nyt++] = 1;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            double sum = 0.0;
+            for (int a = 0; a < nyt; ++a)
+                for (int b = 0; b < nxt; ++b)
+                    sum += part[((yd[a] * 2 + xd[b]) * S + s) * kCells + yc[a] * kCellTile + xc[b]];
+            if (s0 + s < A.nSlices)
+                store_f(static_cast<TO *>(A.img), (size_t)(s0 + s) * A.img_stride + (size_t)py * A.nX + px,
+                        (float)(sum * (double)A.scale));
+        }
+    }
+    __syncthreads();  // part is free for the caller's next tile
+}
+
+template <int S, typename TI, typename TO>
+__global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_kernel(FpaArgs A)
+{
+    __shared__ double part[4 * S * kCellTile * kCellTile];
+    cell_tile<S, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S, part);
+}
+
+// Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, tile), items
+// of slice s are [prefix[s], prefix[s+1]).  A slice whose samples stay in the cells
+// [-1, n-1] takes 15 x 15-pixel cell tiles, as a single-slice call does; any other slice
+// TX x (256/TX) gather tiles.  A fixed grid strides over all items.
 template <int TX, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const FpaSliceDesc *__restrict__ desc,
                                                                        const int *__restrict__ prefix, int n,
                                                                        int total, const TI *sino, TO *img)
 {
+    __shared__ double part[4 * kCellTile * kCellTile];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
         while (lo < hi) {
@@ -394,8 +520,13 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
         A.far_x = d.far_x;
         A.far_y = d.far_y;
         const int local = item - prefix[lo];
-        const int ntx = (A.nX + TX - 1) / TX;
-        adjoint_tile<1, TX, TI, TO>(A, local % ntx, local / ntx, 0);
+        if (adj_cells_fit(A.far_x, A.far_y)) {  // same test as the single-slice launch
+            const int ntx = adj_cell_tiles(A.nX);
+            cell_tile<1, TI, TO>(A, local % ntx, local / ntx, 0, part);
+        } else {
+            const int ntx = (A.nX + TX - 1) / TX;
+            adjoint_tile<1, TX, TI, TO>(A, local % ntx, local / ntx, 0);
+        }
     }
 }
 
@@ -404,10 +535,14 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
 template <int S, typename TI, typename TO>
 static void launch_fpa_k(dim3 grid, hipStream_t stream, const FpaArgs &A)
 {
-    if constexpr (S >= 4)
+    if constexpr (S >= 4) {
         hipLaunchKernelGGL((fp_adjoint_staged_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
-    else
+    } else if (adj_cells_fit(A.far_x, A.far_y)) {
+        const dim3 cg(adj_cell_tiles(A.nX), adj_cell_tiles(A.nY), grid.z);
+        hipLaunchKernelGGL((fp_adjoint_cell_kernel<S, TI, TO>), cg, dim3(kAdjThreads), 0, stream, A);
+    } else {
         hipLaunchKernelGGL((fp_adjoint_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
+    }
 }
 
 template <int S>
