@@ -356,31 +356,108 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_kernel(FpaArgs A)
 }
 
 // ---------------------------------------------------------------- cell-driven variant
-// For one or two slices the per-pixel gather evaluates each sample once for every pixel
-// of its 2x2 support.  Here a thread owns a texel cell (ci, cj), the square between the
-// pixel centres ci..ci+1 and cj..cj+1 (samples inside it interpolate between those four
-// pixels).  Per view and candidate ray (lines within half a cell diagonal of the cell
-// centre) it finds the ray's samples inside the cell: along each axis the samples before
-// boundary b number ceil((b - u0) / du) (du > 0; floor + 1 for du < 0), evaluated by the
-// same expression in both cells that share b, so the cells partition every ray's samples
-// exactly.  The four bilinear weight sums of those n samples (fractions ta + j du,
-// tm + j dw) are closed-form, as in the forward kernel's cell walk, and accumulate into
-// four corner partials.  Corner partials are combined per pixel through LDS at the end,
-// clamp-to-edge folding the outer corners of cells -1 and n-1 into the border pixels.
-// Valid while every sample lies in the cells [-1, n-1] of both axes (2 rFOV/dx - n + 1/2
-// < 1, square images); otherwise the gather kernels run.
+// The per-pixel gather evaluates each sample once for every pixel of its 2x2 support.
+// Here a thread owns a texel cell (ci, cj), the square between the pixel centres
+// ci..ci+1 and cj..cj+1 (samples inside it interpolate between those four pixels).  Per
+// view and candidate ray (lines within half a cell diagonal of the cell centre) it finds
+// the ray's samples inside the cell: along each axis the samples before boundary b number
+// ceil((b - u0) / du) (du > 0; floor + 1 for du < 0), evaluated by the same expression in
+// both cells that share b, so the cells partition every ray's samples exactly.  The four
+// bilinear weight sums of those n samples (fractions ta + j du, tm + j dw) are
+// closed-form, as in the forward kernel's cell walk, and accumulate (FP32) into four
+// corner partials per slice.  Corner partials are combined per pixel through LDS at the
+// end, clamp-to-edge folding the outer corners of cells -1 and n-1 into the border
+// pixels.  Valid while every sample lies in the cells [-1, n-1] of both axes
+// (2 rFOV/dx - n + 1/2 < 1, square images); otherwise the gather kernels run.  Every
+// variant below evaluates a slice with the same arithmetic in the same order, so results
+// do not depend on how many slices share a launch.
 constexpr int kCellTile = kAdjCellTile;  // cells per tile side; a tile yields 15 x 15 pixels
+constexpr int kCellOut = kCellTile - 1;
+constexpr int kCells = kCellTile * kCellTile;  // one per thread
+constexpr int kCombineS = 4;                   // slices combined per LDS pass
+constexpr float kCellRho = 0.70710678f;        // half a cell diagonal (texels)
+
+// Bilinear weight sums of the samples of ray g = (u0, w0, du, dw) (inv = (1/du, 1/dw), N
+// samples) that lie in cell (fci, fcj); w[] for corners (0,0) (1,0) (0,1) (1,1).
+__device__ __forceinline__ void cell_weights(float4 g, float2 inv, int N, float fci, float fcj, float w[4])
+{
+    // sample range inside the cell, per axis, then intersected with [0, N)
+    const float qu0 = (fci - g.x) * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
+    const float qw0 = (fcj - g.y) * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
+    const bool pu = g.z >= 0.0f, pw = g.w >= 0.0f;
+    const float lou = pu ? ceilf(qu0) : floorf(qu1) + 1.0f, hiu = pu ? ceilf(qu1) : floorf(qu0) + 1.0f;
+    const float low = pw ? ceilf(qw0) : floorf(qw1) + 1.0f, hiw = pw ? ceilf(qw1) : floorf(qw0) + 1.0f;
+    const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), (float)N);
+    const float khi = fminf(fminf(hiu, hiw), (float)N);
+    const float n = fmaxf(khi - klo, 0.0f);
+    // fractions of the first sample: the cell index is subtracted before the step is
+    // added, so the FP32 rounding is that of the distance, not of the texel coordinate
+    const float ta = fmaf(klo, g.z, g.x - fci), tm = fmaf(klo, g.w, g.y - fcj);
+    const float t1 = 0.5f * fmaf(n, n, -n);                    // sum of j < n
+    const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);  // sum of j^2
+    const float sa = fmaf(g.z, t1, n * ta);                      // sum of u-fractions
+    const float sm = fmaf(g.w, t1, n * tm);                      // sum of w-fractions
+    const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, g.w, tm * g.z), t1, g.z * g.w * t2));
+    w[0] = (n - sa) - (sm - sam);
+    w[1] = sa - sam;
+    w[2] = sm - sam;
+    w[3] = sam;
+}
+
+// Combines the corner partials acc[4][S] of the tile's cells into its 15 x 15 pixels of
+// slices [s0, s0 + S), kCombineS slices per pass through part (4 x kCombineS x kCells
+// floats of LDS, free when called; every thread calls it).  Pixel (px, py) collects
+// corner dx of cell px - dx (dx = 0, 1), and, on the borders, the clamped outer corner
+// of cell -1 (pixel 0) or of cell n-1 (pixel n-1).
+template <int S, typename TO>
+__device__ __forceinline__ void cell_combine(const FpaArgs &A, float *part, const float (&acc)[4][S], int tx, int ty,
+                                             int s0)
+{
+    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    const int px = tx * kCellOut + lx, py = ty * kCellOut + ly;
+    const bool out = lx < kCellOut && ly < kCellOut && px < A.nX && py < A.nY;
+    int xc[3], xd[3], yc[3], yd[3];  // local cell column and corner of each x (y) term
+    int nxt = 0, nyt = 0;
+    xc[nxt] = lx + 1, xd[nxt++] = 0;
+    xc[nxt] = lx, xd[nxt++] = 1;
+    if (px == 0) xc[nxt] = lx, xd[nxt++] = 0;
+    if (px == A.nX - 1) xc[nxt] = lx + 1, xd[nxt++] = 1;
+    yc[nyt] = ly + 1, yd[nyt++] = 0;
+    yc[nyt] = ly, yd[nyt++] = 1;
+    if (py == 0) yc[nyt] = ly, yd[nyt++] = 0;
+    if (py == A.nY - 1) yc[nyt] = ly + 1, yd[nyt++] = 1;
+#pragma unroll
+    for (int sb = 0; sb < S; sb += kCombineS) {
+        constexpr int kC = S < kCombineS ? S : kCombineS;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < kC; ++k) part[(q * kC + k) * kCells + threadIdx.x] = acc[q][sb + k];
+        __syncthreads();
+        if (out) {
+#pragma unroll
+            for (int k = 0; k < kC; ++k) {
+                double sum = 0.0;
+                for (int a = 0; a < nyt; ++a)
+                    for (int b = 0; b < nxt; ++b)
+                        sum += (double)part[((yd[a] * 2 + xd[b]) * kC + k) * kCells + yc[a] * kCellTile + xc[b]];
+                if (s0 + sb + k < A.nSlices)
+                    store_f(static_cast<TO *>(A.img), (size_t)(s0 + sb + k) * A.img_stride + (size_t)py * A.nX + px,
+                            (float)(sum * (double)A.scale));
+            }
+        }
+        __syncthreads();  // part is free again
+    }
+}
 
 // Cell tile (tx, ty): cells tx*15-1 .. tx*15+14 (and likewise in y), output pixels
-// tx*15 .. tx*15+14.  Every thread of the workgroup calls it (two barriers inside);
-// part holds 4 * S * 256 doubles of LDS.
+// tx*15 .. tx*15+14, slices [s0, s0 + S), every ray read from global memory.  Every
+// thread of the workgroup calls it; part: 4 x min(S, kCombineS) x kCells floats of LDS.
 template <int S, typename TI, typename TO>
-__device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int s0, double *part)
+__device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int s0, float *part)
 {
-    constexpr int kOut = kCellTile - 1;
-    constexpr int kCells = kCellTile * kCellTile;
     const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
-    const int ci = tx * kOut - 1 + lx, cj = ty * kOut - 1 + ly;  // >= -1
+    const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
     const float fci = (float)ci, fcj = (float)cj;
     // cell centre in mm: texel coordinate ci + 1/2 (texel u sits at (u + 1/2 - r) dx)
     const float x = (fci + 1.0f - A.r) * A.dx;
@@ -389,17 +466,16 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
 #pragma unroll
     for (int s = 0; s < S; ++s)
         sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
-    double acc[4][S];
-    float blk[4][S];
+    float acc[4][S];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[q][s] = 0.0, blk[q][s] = 0.0f;
+        for (int s = 0; s < S; ++s) acc[q][s] = 0.0f;
 
     if (ci <= A.nX - 1 && cj <= A.nY - 1) {
         for (int v = 0; v < A.nViews; ++v) {
             int clo, chi;
-            candidate_rays(A, A.cs[v], x, y, 0.70710678f, clo, chi);
+            candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
             const size_t rbase = (size_t)v * A.nCols;
             for (int c = clo; c <= chi; ++c) {
                 const int N = A.ray_n[rbase + c];
@@ -409,73 +485,157 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
                 float sv[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
-                // sample range inside the cell, per axis, then intersected with [0, N)
-                const float qu0 = (fci - g.x) * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
-                const float qw0 = (fcj - g.y) * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
-                const bool pu = g.z >= 0.0f, pw = g.w >= 0.0f;
-                const float lou = pu ? ceilf(qu0) : floorf(qu1) + 1.0f, hiu = pu ? ceilf(qu1) : floorf(qu0) + 1.0f;
-                const float low = pw ? ceilf(qw0) : floorf(qw1) + 1.0f, hiw = pw ? ceilf(qw1) : floorf(qw0) + 1.0f;
-                const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), (float)N);
-                const float khi = fminf(fminf(hiu, hiw), (float)N);
-                const float n = fmaxf(khi - klo, 0.0f);
-                const float ta = fmaf(klo, g.z, g.x) - fci, tm = fmaf(klo, g.w, g.y) - fcj;
-                const float t1 = 0.5f * fmaf(n, n, -n);                    // sum of j < n
-                const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);  // sum of j^2
-                const float sa = fmaf(g.z, t1, n * ta);                      // sum of u-fractions
-                const float sm = fmaf(g.w, t1, n * tm);                      // sum of w-fractions
-                const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, g.w, tm * g.z), t1, g.z * g.w * t2));
-                const float w[4] = {(n - sa) - (sm - sam), sa - sam, sm - sam, sam};  // corners (0,0) (1,0) (0,1) (1,1)
+                float w[4];
+                cell_weights(g, inv, N, fci, fcj, w);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
 #pragma unroll
-                    for (int s = 0; s < S; ++s) blk[q][s] = fmaf(w[q], sv[s], blk[q][s]);
-            }
-            if (v % kViewBlock == kViewBlock - 1 || v == A.nViews - 1) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int s = 0; s < S; ++s) acc[q][s] += (double)blk[q][s], blk[q][s] = 0.0f;
+                    for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int s = 0; s < S; ++s) part[(q * S + s) * kCells + threadIdx.x] = acc[q][s];
-    __syncthreads();
-    // pixel (px, py) collects corner dx of cell px - dx (dx = 0, 1), and, on the borders,
-    // the clamped outer corner of cell -1 (pixel 0) or of cell n-1 (pixel n-1)
-    const int px = tx * kOut + lx, py = ty * kOut + ly;
-    if (lx < kOut && ly < kOut && px < A.nX && py < A.nY) {
-        int xc[3], xd[3], yc[3], yd[3];  // local cell column and corner of each x (y) term
-        int nxt = 0, nyt = 0;
-        xc[nxt] = lx + 1, xd[nxt++] = 0;
-        xc[nxt] = lx, xd[nxt++] = 1;
-        if (px == 0) xc[nxt] = lx, xd[nxt++] = 0;
-        if (px == A.nX - 1) xc[nxt] = lx + 1, xd[nxt++] = 1;
-        yc[nyt] = ly + 1, yd[nyt++] = 0;
-        yc[nyt] = ly, yd[nyt++] = 1;
-        if (py == 0) yc[nyt] = ly, yd[nyt++] = 0;
-        if (py == A.nY - 1) yc[nyt] = ly + 1, yd[nyt++] = 1;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            double sum = 0.0;
-            for (int a = 0; a < nyt; ++a)
-                for (int b = 0; b < nxt; ++b)
-                    sum += part[((yd[a] * 2 + xd[b]) * S + s) * kCells + yc[a] * kCellTile + xc[b]];
-            if (s0 + s < A.nSlices)
-                store_f(static_cast<TO *>(A.img), (size_t)(s0 + s) * A.img_stride + (size_t)py * A.nX + px,
-                        (float)(sum * (double)A.scale));
-        }
-    }
-    __syncthreads();  // part is free for the caller's next tile
+    cell_combine<S, TO>(A, part, acc, tx, ty, s0);
 }
 
 template <int S, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_kernel(FpaArgs A)
 {
-    __shared__ double part[4 * S * kCellTile * kCellTile];
+    __shared__ float part[4 * (S < kCombineS ? S : kCombineS) * kCells];
     cell_tile<S, TI, TO>(A, blockIdx.x, blockIdx.y, blockIdx.z * S, part);
+}
+
+// Staged cell tile for S >= 4 slices: the sinogram columns around the tile's candidate
+// rays are staged in LDS per batch of kStageVB views, as in the staged gather (window
+// from the tile's four corner cells); a candidate column outside the window is read from
+// global memory.
+template <int S>
+union CellShared {
+    AdjStage<S> stage;
+    float part[4 * kCombineS * kCells];
+};
+
+template <int S, typename TI, typename TO>
+__device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S> &sh, int tx, int ty, int s0)
+{
+    static_assert(S % 4 == 0, "a staged column is read four slices at a time");
+    constexpr int VB = kStageVB;
+    constexpr int SQ = S / 4;  // slices staged per wave
+    AdjStage<S> &st = sh.stage;
+    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
+    const bool active = ci <= A.nX - 1 && cj <= A.nY - 1;  // no early exit: every thread stages and waits
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const float fci = (float)ci, fcj = (float)cj;
+    const float x = (fci + 1.0f - A.r) * A.dx;
+    const float y = (fcj + 1.0f - A.r) * A.dx;
+    // corner cell of this lane for the window: lane 4m + c evaluates corner c
+    const int wci = (lane & 1) ? min(tx * kCellOut + kCellOut - 1, A.nX - 1) : tx * kCellOut - 1;
+    const int wcj = (lane & 2) ? min(ty * kCellOut + kCellOut - 1, A.nY - 1) : ty * kCellOut - 1;
+    const float cx = ((float)wci + 1.0f - A.r) * A.dx;
+    const float cy = ((float)wcj + 1.0f - A.r) * A.dx;
+    // this wave stages slices [s0 + wave SQ, s0 + (wave + 1) SQ)
+    const TI *sino_w[SQ];
+#pragma unroll
+    for (int q = 0; q < SQ; ++q)
+        sino_w[q] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + wave * SQ + q, A.nSlices - 1) * A.sino_stride;
+    const TI *sino[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+
+    float acc[4][S];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[q][s] = 0.0f;
+
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
+        int c_lo[VB];
+        {
+            int wlo, whi;
+            candidate_rays(A, A.cs[min(v0 + (lane >> 2) % VB, A.nViews - 1)], cx, cy, kCellRho, wlo, whi);
+#pragma unroll
+            for (int sh2 = 1; sh2 <= 2; sh2 <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh2));
+#pragma unroll
+            for (int m = 0; m < VB; ++m)
+                c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kStageCols, 0));
+        }
+        __syncthreads();  // the previous batch's reads are done
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            const int c = c_lo[m] + lane;
+            if (v0 + m >= A.nViews || c >= A.nCols) continue;
+            const size_t rb = (size_t)(v0 + m) * A.nCols + c;
+            const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+            float q4[SQ];
+#pragma unroll
+            for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(sino_w[q], boff);
+            float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
+            if constexpr (SQ % 4 == 0) {
+#pragma unroll
+                for (int q = 0; q < SQ; q += 4)
+                    *reinterpret_cast<float4 *>(dst + q) = make_float4(q4[q], q4[q + 1], q4[q + 2], q4[q + 3]);
+            } else if constexpr (SQ == 2) {
+                *reinterpret_cast<float2 *>(dst) = make_float2(q4[0], q4[1]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
+            }
+            if (wave == 0) st.geo[m][lane] = A.ray_geo[rb];
+            else if (wave == 1) st.inv[m][lane] = A.ray_inv[rb];
+            else if (wave == 2) st.n[m][lane] = A.ray_n[rb];
+        }
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll 1
+        for (int m = 0; m < VB; ++m) {
+            const int v = v0 + m;
+            if (v >= A.nViews) break;
+            int clo, chi;
+            candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
+            for (int c = clo; c <= chi; ++c) {
+                const int lc = c - c_lo[m];
+                int N;
+                float4 g;
+                float2 inv;
+                float sv[S];
+                if ((unsigned)lc < (unsigned)kStageCols) {
+                    N = st.n[m][lc];
+                    g = st.geo[m][lc];
+                    inv = st.inv[m][lc];
+#pragma unroll
+                    for (int s = 0; s < S; s += 4) {
+                        const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
+                        sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
+                    }
+                } else {
+                    const size_t rb = (size_t)v * A.nCols + c;
+                    N = A.ray_n[rb];
+                    g = A.ray_geo[rb];
+                    inv = A.ray_inv[rb];
+                    const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+                }
+                float w[4];
+                cell_weights(g, inv, N, fci, fcj, w);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
+            }
+        }
+    }
+    __syncthreads();  // staging reads are done: the union turns into corner partials
+    cell_combine<S, TO>(A, sh.part, acc, tx, ty, s0);
+}
+
+template <int S, typename TI, typename TO>
+__global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_staged_kernel(FpaArgs A)
+{
+    __shared__ CellShared<S> sh;
+    cell_staged_tile<S, TI, TO>(A, sh, blockIdx.x, blockIdx.y, blockIdx.z * S);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, tile), items
@@ -487,7 +647,7 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
                                                                        const int *__restrict__ prefix, int n,
                                                                        int total, const TI *sino, TO *img)
 {
-    __shared__ double part[4 * kCellTile * kCellTile];
+    __shared__ float part[4 * kCells];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
         while (lo < hi) {
@@ -535,11 +695,14 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
 template <int S, typename TI, typename TO>
 static void launch_fpa_k(dim3 grid, hipStream_t stream, const FpaArgs &A)
 {
-    if constexpr (S >= 4) {
-        hipLaunchKernelGGL((fp_adjoint_staged_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
-    } else if (adj_cells_fit(A.far_x, A.far_y)) {
+    if (adj_cells_fit(A.far_x, A.far_y)) {
         const dim3 cg(adj_cell_tiles(A.nX), adj_cell_tiles(A.nY), grid.z);
-        hipLaunchKernelGGL((fp_adjoint_cell_kernel<S, TI, TO>), cg, dim3(kAdjThreads), 0, stream, A);
+        if constexpr (S >= 4)
+            hipLaunchKernelGGL((fp_adjoint_cell_staged_kernel<S, TI, TO>), cg, dim3(kAdjThreads), 0, stream, A);
+        else
+            hipLaunchKernelGGL((fp_adjoint_cell_kernel<S, TI, TO>), cg, dim3(kAdjThreads), 0, stream, A);
+    } else if constexpr (S >= 4) {
+        hipLaunchKernelGGL((fp_adjoint_staged_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
     } else {
         hipLaunchKernelGGL((fp_adjoint_kernel<S, TI, TO>), grid, dim3(kAdjThreads), 0, stream, A);
     }
