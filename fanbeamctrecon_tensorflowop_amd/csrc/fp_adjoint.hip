@@ -404,6 +404,19 @@ __device__ __forceinline__ void cell_weights(float4 g, float2 inv, int N, float 
     w[3] = sam;
 }
 
+// Adds the S-slice contribution of one ray to the cell's corner partials.
+template <int S>
+__device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, int N, const float (&sv)[S], float fci, float fcj,
+                                                float (&acc)[4][S])
+{
+    float w[4];
+    cell_weights(g, inv, N, fci, fcj, w);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
+}
+
 // Combines the corner partials acc[4][S] of the tile's cells into its 15 x 15 pixels of
 // slices [s0, s0 + S), kCombineS slices per pass through part (4 x kCombineS x kCells
 // floats of LDS, free when called; every thread calls it).  Pixel (px, py) collects
@@ -485,12 +498,7 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
                 float sv[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
-                float w[4];
-                cell_weights(g, inv, N, fci, fcj, w);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
+                cell_accumulate<S>(g, inv, N, sv, fci, fcj, acc);
             }
         }
     }
@@ -506,11 +514,27 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_kernel(FpaArgs A)
 
 // Staged cell tile for S >= 4 slices: the sinogram columns around the tile's candidate
 // rays are staged in LDS per batch of kStageVB views, as in the staged gather (window
-// from the tile's four corner cells); a candidate column outside the window is read from
-// global memory.
+// from the tile's four corner cells), each column's ray tables packed in one 32-byte
+// record.  Per view the candidate columns run in three ascending stretches: left of the
+// window (global memory), inside it (LDS only) and right of it (global memory), so the
+// hot middle loop carries no global addressing and no branch.
+struct CellRay {
+    float4 geo;
+    float2 inv;
+    int n;
+    int pad;
+};
+
+template <int S>
+struct CellStage {
+    static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
+    float val[kStageVB][kStageCols][kPitch];
+    CellRay ray[kStageVB][kStageCols];
+};
+
 template <int S>
 union CellShared {
-    AdjStage<S> stage;
+    CellStage<S> stage;
     float part[4 * kCombineS * kCells];
 };
 
@@ -520,7 +544,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     static_assert(S % 4 == 0, "a staged column is read four slices at a time");
     constexpr int VB = kStageVB;
     constexpr int SQ = S / 4;  // slices staged per wave
-    AdjStage<S> &st = sh.stage;
+    CellStage<S> &st = sh.stage;
     const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
     const bool active = ci <= A.nX - 1 && cj <= A.nY - 1;  // no early exit: every thread stages and waits
@@ -534,21 +558,37 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     const int wcj = (lane & 2) ? min(ty * kCellOut + kCellOut - 1, A.nY - 1) : ty * kCellOut - 1;
     const float cx = ((float)wci + 1.0f - A.r) * A.dx;
     const float cy = ((float)wcj + 1.0f - A.r) * A.dx;
-    // this wave stages slices [s0 + wave SQ, s0 + (wave + 1) SQ)
-    const TI *sino_w[SQ];
-#pragma unroll
-    for (int q = 0; q < SQ; ++q)
-        sino_w[q] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + wave * SQ + q, A.nSlices - 1) * A.sino_stride;
-    const TI *sino[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-        sino[s] = static_cast<const TI *>(A.sino) + (size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride;
+    // the group's slices are addressed from its first one by 32-bit byte offsets (the
+    // launch checks that S sinograms span less than 4 GiB); a ragged last group repeats
+    // its last slice
+    const TI *gbase = static_cast<const TI *>(A.sino) + (size_t)s0 * A.sino_stride;
+    const int last = A.nSlices - 1 - s0;
+    const unsigned sbytes = (unsigned)A.sino_stride * (unsigned)sizeof(TI);
 
     float acc[4][S];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[q][s] = 0.0f;
+
+    // a candidate column outside the staged window (rare): tables and values from global
+    // memory, one slice at a time so that the path holds few registers (the scheduling
+    // barrier keeps the loads from being hoisted together); the same FMAs as
+    // cell_accumulate
+    auto global_ray = [&](int v, int c) {
+        const size_t rb = (size_t)v * A.nCols + c;
+        float w[4];
+        cell_weights(A.ray_geo[rb], A.ray_inv[rb], A.ray_n[rb], fci, fcj, w);
+        unsigned off = (unsigned)rb * (unsigned)sizeof(TI);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const float val = load_f_at(gbase, off);
+            if (s < last) off += sbytes;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q][s] = fmaf(w[q], val, acc[q][s]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
 
     for (int v0 = 0; v0 < A.nViews; v0 += VB) {
         int c_lo[VB];
@@ -570,7 +610,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
             float q4[SQ];
 #pragma unroll
-            for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(sino_w[q], boff);
+            for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
             float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
             if constexpr (SQ % 4 == 0) {
 #pragma unroll
@@ -582,9 +622,9 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
 #pragma unroll
                 for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
             }
-            if (wave == 0) st.geo[m][lane] = A.ray_geo[rb];
-            else if (wave == 1) st.inv[m][lane] = A.ray_inv[rb];
-            else if (wave == 2) st.n[m][lane] = A.ray_n[rb];
+            if (wave == 0) st.ray[m][lane].geo = A.ray_geo[rb];
+            else if (wave == 1) st.ray[m][lane].inv = A.ray_inv[rb];
+            else if (wave == 2) st.ray[m][lane].n = A.ray_n[rb];
         }
         __syncthreads();
         if (!active) continue;
@@ -594,37 +634,21 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             if (v >= A.nViews) break;
             int clo, chi;
             candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
-            for (int c = clo; c <= chi; ++c) {
-                const int lc = c - c_lo[m];
-                int N;
-                float4 g;
-                float2 inv;
+            const int wlo = c_lo[m], whi = c_lo[m] + kStageCols - 1;  // staged window
+            int c = clo;
+            for (; c <= min(chi, wlo - 1); ++c) global_ray(v, c);
+            for (; c <= min(chi, whi); ++c) {
+                const int lc = c - wlo;
+                const CellRay &rr = st.ray[m][lc];
                 float sv[S];
-                if ((unsigned)lc < (unsigned)kStageCols) {
-                    N = st.n[m][lc];
-                    g = st.geo[m][lc];
-                    inv = st.inv[m][lc];
 #pragma unroll
-                    for (int s = 0; s < S; s += 4) {
-                        const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
-                        sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
-                    }
-                } else {
-                    const size_t rb = (size_t)v * A.nCols + c;
-                    N = A.ray_n[rb];
-                    g = A.ray_geo[rb];
-                    inv = A.ray_inv[rb];
-                    const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
-#pragma unroll
-                    for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+                for (int s = 0; s < S; s += 4) {
+                    const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
+                    sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
                 }
-                float w[4];
-                cell_weights(g, inv, N, fci, fcj, w);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
+                cell_accumulate<S>(rr.geo, rr.inv, rr.n, sv, fci, fcj, acc);
             }
+            for (; c <= chi; ++c) global_ray(v, c);
         }
     }
     __syncthreads();  // staging reads are done: the union turns into corner partials
@@ -695,7 +719,7 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
 template <int S, typename TI, typename TO>
 static void launch_fpa_k(dim3 grid, hipStream_t stream, const FpaArgs &A)
 {
-    if (adj_cells_fit(A.far_x, A.far_y)) {
+    if (adj_cells_fit(A.far_x, A.far_y) && (S < 4 || (long long)S * A.sino_stride * (long long)sizeof(TI) < (1LL << 32))) {
         const dim3 cg(adj_cell_tiles(A.nX), adj_cell_tiles(A.nY), grid.z);
         if constexpr (S >= 4)
             hipLaunchKernelGGL((fp_adjoint_cell_staged_kernel<S, TI, TO>), cg, dim3(kAdjThreads), 0, stream, A);
