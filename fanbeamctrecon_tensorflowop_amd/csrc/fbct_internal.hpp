@@ -13,13 +13,17 @@
 namespace fbct {
 
 // FP ray-march tiling (see fp.hip).  Bundle configurations: rays per workgroup (lanes
-// per ray = kFpThreads / rays) and slab width in pixels along the major axis.  The
-// 32-ray, 20-pixel configuration serves the 4-slice tiles: its smaller LDS tile keeps
-// five workgroups resident per CU.
+// per ray = kFpThreads / rays) and slab width in pixels along the major axis (a multiple
+// of the lanes per ray, so every lane's share of a slab ends on a texel boundary).  The
+// 32-ray, 24-pixel configuration serves the 4-slice tiles: its smaller LDS tile keeps
+// five to seven workgroups resident per CU.
 constexpr int kFpThreads = 256;
+// Workgroups per CU the FP kernels can hold by their registers (4-slice tile kernel: 71
+// VGPRs, 7 waves per SIMD); the LDS tile is sized for this many where the work allows.
+constexpr int kFpWgsPerCu = 7;
 constexpr int kFpNumCfg = 2;
 constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
-constexpr int kFpCfgSlab[kFpNumCfg] = {32, 20};
+constexpr int kFpCfgSlab[kFpNumCfg] = {32, 24};
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each with its own
 // L2: the workgroup with linear id b runs on XCD b % 8.  xcd_block maps b to a logical
@@ -99,6 +103,7 @@ struct DeviceTables {
     uint8_t *fp_major[kFpNumCfg] = {};  // per bundle config: [nViews*nb] 1 = x-major
     short2 *fp_slabs[kFpNumCfg] = {};   // per bundle config: [nViews*nb] first and last slab
     short2 *fp_rows[kFpNumCfg] = {};    // per bundle config: [nViews*nb*nslab] tile rows [lo, hi]
+    int32_t *fp_order[kFpNumCfg] = {};  // per bundle config: items v*nb+b by decreasing tile rows
     float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
     double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
     double *ramp = nullptr;        // [2*nCols-1]
@@ -142,6 +147,10 @@ struct fbct_plan {
     std::vector<uint8_t> fp_major[fbct::kFpNumCfg];
     std::vector<int16_t> fp_slabs[fbct::kFpNumCfg];  // 2 per bundle
     std::vector<int16_t> fp_rows[fbct::kFpNumCfg];   // 2 per (bundle, slab)
+    // items (v * nb + b) ordered by decreasing largest tile row count, and those counts:
+    // the launch gives the items that fit a smaller tile a launch of their own
+    std::vector<int32_t> fp_order[fbct::kFpNumCfg];
+    std::vector<int16_t> fp_order_rows[fbct::kFpNumCfg];
     std::vector<float> view_cs;     // 2 per view
     std::vector<double> view_csd;   // 2 per view
     std::vector<double> ramp;       // 2*nCols-1

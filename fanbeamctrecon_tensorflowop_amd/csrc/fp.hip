@@ -149,7 +149,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     const size_t vb = (size_t)v * A.nb + b;
     const short2 srange = A.slabs[vb];
     const short2 *rows = A.rows + vb * A.nslab;
-    int kprev = slab_boundary(a0, da, ia, N, (float)(srange.x * kSlab - 1));
+    // lane l of a ray takes the samples whose major coordinate lies in
+    // [alo + l w, alo + (l + 1) w), w = kSlab / G: it finds the end of its share and takes
+    // the start from lane l - 1 (lane 0: the previous slab's end), so the G shares
+    // partition the slab's samples and a share ends on a texel boundary whenever l w is
+    // an integer
+    constexpr float kShare = (float)kSlab / (float)G;
+    int kcarry = slab_boundary(a0, da, ia, N, (float)(srange.x * kSlab - 1));
     // byte strides of the interleaved tile
     constexpr float kColB = 4.0f * S;
     constexpr float kRowB = 4.0f * S * kPitch;
@@ -159,13 +165,15 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 
     for (int sl = srange.x; sl <= srange.y; ++sl) {
         const int alo = sl * kSlab - 1;
-        const int knext = slab_boundary(a0, da, ia, N, (float)(alo + kSlab));
-        const int ka = asc ? kprev : knext;
-        const int kb = asc ? knext : kprev;
-        kprev = knext;
-
         const short2 rr = rows[sl];  // uniform
         const int mlo = rr.x, mhi = rr.y;
+        const float tend = lane == G - 1 ? (float)(alo + kSlab) : fmaf((float)(lane + 1), kShare, (float)alo);
+        const int kown = slab_boundary(a0, da, ia, N, tend);
+        // lane l - 1's end (DPP row_shr:1; lane 0 of a ray takes the carry instead), and
+        // for the next slab's lane 0 the end of lane G - 1 (swizzle lane ^ (G - 1))
+        int kstart = __builtin_amdgcn_update_dpp(0, kown, 0x111, 0xf, 0xf, false);
+        if (lane == 0) kstart = kcarry;
+        kcarry = __builtin_amdgcn_ds_swizzle(kown, 0x1f | ((G - 1) << 10));
         if (mlo > mhi) continue;  // no samples of this bundle in the slab
         const int M = mhi - mlo + 1;
         if (M > A.cap) {  // cannot happen for tables built by the plan; never read past LDS
@@ -207,15 +215,14 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 
         // byte offset of tap (fa, fm) = (fm - mlo) * kRowB + (fa - alo) * kColB, exact in FP32
         const float obase = tbase - ((float)mlo * kRowB + (float)alo * kColB);
-        // The slab's samples [ka, kb) are split into G contiguous shares, one per lane.  All
+        // A lane's share of the slab is [kl, kend) (major-coordinate split above).  All
         // samples inside one texel cell (floor a, floor m) read the same four taps, so a lane
         // walks its share cell by cell: the n samples of a cell have fractions ta + j da and
         // tm + j dm (j < n), their bilinear weights are summed in closed form and the taps
         // are read and weighted once per cell.  A sample on a cell boundary may be counted
         // in either cell: the bilinear interpolant is continuous there.
-        const int len = (kb - ka + G - 1) / G;
-        const int kl = ka + lane * len;
-        const float kend = (float)min(kl + len, kb);
+        const int kl = asc ? kstart : kown;
+        const float kend = (float)(asc ? kown : kstart);
         float acc[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] = 0.0f;
@@ -260,12 +267,15 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     }
 }
 
+// Items order[item0 + t] (t < gridDim.x) of slice group blockIdx.y; item = v * nb + b.
 template <int S, int G, typename TI, typename TO>
-__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, int *err)
+__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, const int32_t *__restrict__ order, int item0,
+                                                        int *err)
 {
-    unsigned t, g;  // XCD-aware (bundle + view tile, slice group)
-    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
-    fp_bundle<S, G, TI, TO>(A, t % gridDim.x, t / gridDim.x, g * S, err);
+    unsigned t, g;  // XCD-aware (item, slice group)
+    xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, t, g);
+    const int vb = order[item0 + (int)t];
+    fp_bundle<S, G, TI, TO>(A, vb % A.nb, vb / A.nb, g * S, err);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, view,
@@ -338,19 +348,20 @@ static fbct_status fp_err_ptr(int **out)
 }
 
 template <int S, int G, typename TI, typename TO>
-static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, int *err)
+static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, const int32_t *order, int item0,
+                        int *err)
 {
-    hipLaunchKernelGGL((fp_kernel<S, G, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, err);
+    hipLaunchKernelGGL((fp_kernel<S, G, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, order, item0, err);
 }
 
 template <int S, int G>
 static fbct_status launch_fp_t(fbct_dtype ti, fbct_dtype to, dim3 grid, size_t lds, hipStream_t stream,
-                               const FpArgs &A, int *err)
+                               const FpArgs &A, const int32_t *order, int item0, int *err)
 {
-    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, G, float, float>(grid, lds, stream, A, err);
-    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, G, bf16_t, float>(grid, lds, stream, A, err);
-    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, G, float, bf16_t>(grid, lds, stream, A, err);
-    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, G, bf16_t, bf16_t>(grid, lds, stream, A, err);
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, G, float, float>(grid, lds, stream, A, order, item0, err);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, G, bf16_t, float>(grid, lds, stream, A, order, item0, err);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, G, float, bf16_t>(grid, lds, stream, A, order, item0, err);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, G, bf16_t, bf16_t>(grid, lds, stream, A, order, item0, err);
     else return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp: unknown storage type");
     return FBCT_OK;
 }
@@ -396,24 +407,42 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.slabs = t.fp_slabs[cfg];
     A.rows = t.fp_rows[cfg];
     A.nb = bc.nb;
-    A.cap = bc.cap;
     A.nslab = bc.nslab;
-    const size_t lds = tile_bytes(cfg, S);
-    if (lds > 160 * 1024)
-        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", lds);
-    dim3 grid(bc.nb, p.g.nViews, (n + S - 1) / S);
-    fbct_status st;
-    if (cfg == 0) {
-        if (S == 2) st = launch_fp_t<2, 4>(ti, to, grid, lds, stream, A, err);
-        else st = launch_fp_t<1, 4>(ti, to, grid, lds, stream, A, err);
-    } else {
-        if (S == 4) st = launch_fp_t<4, 8>(ti, to, grid, lds, stream, A, err);
-        else if (S == 2) st = launch_fp_t<2, 8>(ti, to, grid, lds, stream, A, err);
-        else st = launch_fp_t<1, 8>(ti, to, grid, lds, stream, A, err);
+    if (tile_bytes(cfg, S) > 160 * 1024)
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_fp: slab tile of %zu bytes exceeds LDS", tile_bytes(cfg, S));
+    // The LDS tile sets how many workgroups a CU holds (up to kFpWgsPerCu, where the
+    // registers cap it).  Items whose largest tile fits the rows of that many workgroups
+    // get a launch of their own with that smaller tile; the others (ordered first) a
+    // launch with the plan's full tile.
+    const size_t row_bytes = (size_t)fp_pitch(cfg) * sizeof(float) * S;
+    const int small_cap = (int)std::min<size_t>((size_t)bc.cap, (160 * 1024 / kFpWgsPerCu) / row_bytes);
+    const std::vector<int16_t> &orows = p.fp_order_rows[cfg];
+    const int nitems = (int)orows.size();
+    // first item with at most small_cap rows (counts are non-increasing)
+    const int split = (int)(std::upper_bound(orows.begin(), orows.end(), (int16_t)small_cap,
+                                             [](int16_t cap_v, int16_t r) { return r <= cap_v; }) -
+                            orows.begin());
+    const unsigned groups = (unsigned)((n + S - 1) / S);
+    for (int part = 0; part < 2; ++part) {
+        const int item0 = part == 0 ? 0 : split;
+        const int count = part == 0 ? split : nitems - split;
+        if (count == 0) continue;
+        A.cap = part == 0 ? bc.cap : small_cap;
+        const size_t lds = (size_t)A.cap * row_bytes;
+        const dim3 grid((unsigned)count, groups);
+        fbct_status st;
+        if (cfg == 0) {
+            if (S == 2) st = launch_fp_t<2, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+            else st = launch_fp_t<1, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+        } else {
+            if (S == 4) st = launch_fp_t<4, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+            else if (S == 2) st = launch_fp_t<2, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+            else st = launch_fp_t<1, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+        }
+        if (st != FBCT_OK) return st;
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));
     }
-    if (st != FBCT_OK) return st;
-    e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

@@ -401,6 +401,22 @@ static void build_tables(fbct_plan &p)
         int cap = 1;
         for (int c : cap_of) cap = std::max(cap, c);
         bc.cap = cap;
+        // items by decreasing largest tile (ties: item order)
+        const size_t nitems = static_cast<size_t>(nV) * nb;
+        std::vector<int16_t> item_rows(nitems, 0);
+        for (size_t vb = 0; vb < nitems; ++vb)
+            for (int sl = 0; sl < nslab; ++sl) {
+                const int16_t *e = &rows[2 * (vb * nslab + sl)];
+                if (e[0] <= e[1]) item_rows[vb] = std::max<int16_t>(item_rows[vb], static_cast<int16_t>(e[1] - e[0] + 1));
+            }
+        std::vector<int32_t> &order = p.fp_order[cfg];
+        order.resize(nitems);
+        for (size_t vb = 0; vb < nitems; ++vb) order[vb] = static_cast<int32_t>(vb);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t x, int32_t y) { return item_rows[x] > item_rows[y]; });
+        std::vector<int16_t> &orows = p.fp_order_rows[cfg];
+        orows.resize(nitems);
+        for (size_t k = 0; k < nitems; ++k) orows[k] = item_rows[order[k]];
     }
 
     // BP: xi_raw = (atan(t/U) - dLeft)/dCol + dShift  (BP/transform.cu:85).
@@ -493,6 +509,7 @@ static void free_tables(DeviceTables &t)
         (void)hipFree(t.fp_major[cfg]);
         (void)hipFree(t.fp_slabs[cfg]);
         (void)hipFree(t.fp_rows[cfg]);
+        (void)hipFree(t.fp_order[cfg]);
     }
     (void)hipFree(t.ray_inv);
     (void)hipFree(t.view_cs);
@@ -568,6 +585,8 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
         (s = upload(&t.fp_slabs[1], plan->fp_slabs[1].data(), plan->fp_slabs[1].size() * sizeof(int16_t))) != FBCT_OK ||
         (s = upload(&t.fp_rows[0], plan->fp_rows[0].data(), plan->fp_rows[0].size() * sizeof(int16_t))) != FBCT_OK ||
         (s = upload(&t.fp_rows[1], plan->fp_rows[1].data(), plan->fp_rows[1].size() * sizeof(int16_t))) != FBCT_OK ||
+        (s = upload(&t.fp_order[0], plan->fp_order[0].data(), plan->fp_order[0].size() * sizeof(int32_t))) != FBCT_OK ||
+        (s = upload(&t.fp_order[1], plan->fp_order[1].data(), plan->fp_order[1].size() * sizeof(int32_t))) != FBCT_OK ||
         (s = upload(&t.ray_inv, plan->ray_inv.data(), plan->ray_inv.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_csd, plan->view_csd.data(), plan->view_csd.size() * sizeof(double))) != FBCT_OK ||
