@@ -379,16 +379,18 @@ constexpr float kCellRho = 0.70710678f;        // half a cell diagonal (texels)
 
 // Bilinear weight sums of the samples of ray g = (u0, w0, du, dw) (inv = (1/du, 1/dw), N
 // samples) that lie in cell (fci, fcj); w[] for corners (0,0) (1,0) (0,1) (1,1).
-__device__ __forceinline__ void cell_weights(float4 g, float2 inv, int N, float fci, float fcj, float w[4])
+__device__ __forceinline__ void cell_weights(float4 g, float2 inv, float N, float fci, float fcj, float w[4])
 {
-    // sample range inside the cell, per axis, then intersected with [0, N)
+    // sample range inside the cell, per axis, then intersected with [0, N): the samples
+    // that precede boundary b along the ray number ceil((b - u0) / du) in either
+    // direction (a sample exactly on b goes to the cell it leaves, where its bilinear
+    // weights are the same), so the range is [ceil(min q), ceil(max q))
     const float qu0 = (fci - g.x) * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
     const float qw0 = (fcj - g.y) * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
-    const bool pu = g.z >= 0.0f, pw = g.w >= 0.0f;
-    const float lou = pu ? ceilf(qu0) : floorf(qu1) + 1.0f, hiu = pu ? ceilf(qu1) : floorf(qu0) + 1.0f;
-    const float low = pw ? ceilf(qw0) : floorf(qw1) + 1.0f, hiw = pw ? ceilf(qw1) : floorf(qw0) + 1.0f;
-    const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), (float)N);
-    const float khi = fminf(fminf(hiu, hiw), (float)N);
+    const float lou = ceilf(fminf(qu0, qu1)), hiu = ceilf(fmaxf(qu0, qu1));
+    const float low = ceilf(fminf(qw0, qw1)), hiw = ceilf(fmaxf(qw0, qw1));
+    const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), N);
+    const float khi = fminf(fminf(hiu, hiw), N);
     const float n = fmaxf(khi - klo, 0.0f);
     // fractions of the first sample: the cell index is subtracted before the step is
     // added, so the FP32 rounding is that of the distance, not of the texel coordinate
@@ -406,8 +408,8 @@ __device__ __forceinline__ void cell_weights(float4 g, float2 inv, int N, float 
 
 // Adds the S-slice contribution of one ray to the cell's corner partials.
 template <int S>
-__device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, int N, const float (&sv)[S], float fci, float fcj,
-                                                float (&acc)[4][S])
+__device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, const float (&sv)[S], float fci,
+                                                float fcj, float (&acc)[4][S])
 {
     float w[4];
     cell_weights(g, inv, N, fci, fcj, w);
@@ -491,7 +493,7 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
             candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
             const size_t rbase = (size_t)v * A.nCols;
             for (int c = clo; c <= chi; ++c) {
-                const int N = A.ray_n[rbase + c];
+                const float N = (float)A.ray_n[rbase + c];
                 const float4 g = A.ray_geo[rbase + c];
                 const float2 inv = A.ray_inv[rbase + c];
                 const unsigned boff = (unsigned)(rbase + c) * (unsigned)sizeof(TI);
@@ -518,18 +520,14 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_kernel(FpaArgs A)
 // record.  Per view the candidate columns run in three ascending stretches: left of the
 // window (global memory), inside it (LDS only) and right of it (global memory), so the
 // hot middle loop carries no global addressing and no branch.
-struct CellRay {
-    float4 geo;
-    float2 inv;
-    int n;
-    int pad;
-};
-
 template <int S>
 struct CellStage {
     static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
     float val[kStageVB][kStageCols][kPitch];
-    CellRay ray[kStageVB][kStageCols];
+    // ray tables of the staged columns: (u0, w0, du, dw) and (1/du, 1/dw, N, -), each a
+    // 16-byte record so that 16 lanes reading distinct columns hit distinct banks
+    float4 geo[kStageVB][kStageCols];
+    float4 aux[kStageVB][kStageCols];
 };
 
 template <int S>
@@ -578,7 +576,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     auto global_ray = [&](int v, int c) {
         const size_t rb = (size_t)v * A.nCols + c;
         float w[4];
-        cell_weights(A.ray_geo[rb], A.ray_inv[rb], A.ray_n[rb], fci, fcj, w);
+        cell_weights(A.ray_geo[rb], A.ray_inv[rb], (float)A.ray_n[rb], fci, fcj, w);
         unsigned off = (unsigned)rb * (unsigned)sizeof(TI);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -622,9 +620,12 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
 #pragma unroll
                 for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
             }
-            if (wave == 0) st.ray[m][lane].geo = A.ray_geo[rb];
-            else if (wave == 1) st.ray[m][lane].inv = A.ray_inv[rb];
-            else if (wave == 2) st.ray[m][lane].n = A.ray_n[rb];
+            if (wave == 0) {
+                st.geo[m][lane] = A.ray_geo[rb];
+            } else if (wave == 1) {
+                const float2 iv = A.ray_inv[rb];
+                st.aux[m][lane] = make_float4(iv.x, iv.y, (float)A.ray_n[rb], 0.0f);
+            }
         }
         __syncthreads();
         if (!active) continue;
@@ -639,14 +640,14 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             for (; c <= min(chi, wlo - 1); ++c) global_ray(v, c);
             for (; c <= min(chi, whi); ++c) {
                 const int lc = c - wlo;
-                const CellRay &rr = st.ray[m][lc];
+                const float4 ax = st.aux[m][lc];
                 float sv[S];
 #pragma unroll
                 for (int s = 0; s < S; s += 4) {
                     const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
                     sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
                 }
-                cell_accumulate<S>(rr.geo, rr.inv, rr.n, sv, fci, fcj, acc);
+                cell_accumulate<S>(st.geo[m][lc], make_float2(ax.x, ax.y), ax.z, sv, fci, fcj, acc);
             }
             for (; c <= chi; ++c) global_ray(v, c);
         }
