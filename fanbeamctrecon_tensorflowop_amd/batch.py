@@ -29,7 +29,7 @@ __all__ = ["UnevenBatch", "StepGraph", "autotune", "FP_RAYS", "ADJ_TILE_X", "WGS
 
 FP_RAYS = (64, 32)
 ADJ_TILE_X = (16, 32, 8, 64)
-WGS_PER_CU = (2, 4, 8, 16)
+WGS_PER_CU = (0, 4, 8, 16, 32)  # 0: one workgroup per work item
 
 
 class UnevenBatch:

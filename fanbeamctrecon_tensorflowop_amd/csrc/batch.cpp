@@ -29,6 +29,7 @@ void free_batch_device(BatchDevice &d)
     (void)hipFree(d.fp_desc);
     (void)hipFree(d.fpa_desc);
     for (int k = 0; k < kFpNumCfg; ++k) (void)hipFree(d.fp_prefix[k]);
+    for (int k = 0; k < kFpNumCfg; ++k) (void)hipFree(d.fp_order[k]);
     for (int k = 0; k < kAdjNumTiles; ++k) (void)hipFree(d.fpa_prefix[k]);
     if (cur >= 0) (void)hipSetDevice(cur);
     d = BatchDevice{};
@@ -99,6 +100,7 @@ fbct_status batch_device(fbct_batch *b, const BatchDevice **out)
     fbct_status st = upload_vec(&d.fp_desc, fpd);
     if (st == FBCT_OK) st = upload_vec(&d.fpa_desc, fpad);
     for (int k = 0; k < kFpNumCfg && st == FBCT_OK; ++k) st = upload_vec(&d.fp_prefix[k], b->fp_prefix[k]);
+    for (int k = 0; k < kFpNumCfg && st == FBCT_OK; ++k) st = upload_vec(&d.fp_order[k], b->fp_order[k]);
     for (int k = 0; k < kAdjNumTiles && st == FBCT_OK; ++k) st = upload_vec(&d.fpa_prefix[k], b->fpa_prefix[k]);
     if (st != FBCT_OK) {
         free_batch_device(d);
@@ -118,9 +120,9 @@ fbct_status check_shape(const fbct_batch_shape &s)
 {
     if (s.fp_rays != 64 && s.fp_rays != 32)
         return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: fp_rays must be 64 or 32 (got %d)", s.fp_rays);
-    if (s.fp_wgs_per_cu < 1 || s.fp_wgs_per_cu > kMaxWgsPerCu || s.adj_wgs_per_cu < 1 ||
+    if (s.fp_wgs_per_cu < 0 || s.fp_wgs_per_cu > kMaxWgsPerCu || s.adj_wgs_per_cu < 0 ||
         s.adj_wgs_per_cu > kMaxWgsPerCu)
-        return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: workgroups per CU must be in 1..%d", kMaxWgsPerCu);
+        return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: workgroups per CU must be in 0..%d", kMaxWgsPerCu);
     for (int k = 0; k < kAdjNumTiles; ++k)
         if (kAdjTileX[k] == s.adj_tile_x) return FBCT_OK;
     return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: adj_tile_x must be 8, 16, 32 or 64 (got %d)", s.adj_tile_x);
@@ -187,7 +189,31 @@ fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch 
             b->fp_samples += p.fp_samples;
             b->updates += (int64_t)p.g.nX * p.g.nY * p.g.nViews;
         }
-        b->shape = fbct_batch_shape{64, 8, 16, 8};
+        // FP items of all slices: those whose largest tile exceeds the rows of
+        // kFpWgsPerCu workgroups per CU first, then the rest, each in item order
+        for (int k = 0; k < fbct::kFpNumCfg; ++k) {
+            const int total = b->fp_prefix[k][n_slices];
+            const size_t row_bytes = static_cast<size_t>(fbct::kFpCfgSlab[k] + 1) * sizeof(float);
+            const int small_cap = static_cast<int>(std::min<size_t>(
+                static_cast<size_t>(b->fp_cap_max[k]), (160 * 1024 / fbct::kFpWgsPerCu) / row_bytes));
+            std::vector<uint8_t> big(static_cast<size_t>(total), 0);
+            for (int s = 0; s < n_slices; ++s) {
+                const fbct_plan &p = *plans[s];
+                const int base = b->fp_prefix[k][s];
+                for (size_t q = 0; q < p.fp_order[k].size(); ++q)
+                    if (p.fp_order_rows[k][q] > small_cap) big[static_cast<size_t>(base + p.fp_order[k][q])] = 1;
+            }
+            std::vector<int32_t> &order = b->fp_order[k];
+            order.clear();
+            order.reserve(static_cast<size_t>(total));
+            for (int g = 0; g < total; ++g)
+                if (big[g]) order.push_back(g);
+            b->fp_split[k] = static_cast<int>(order.size());
+            for (int g = 0; g < total; ++g)
+                if (!big[g]) order.push_back(g);
+            b->fp_small_cap[k] = small_cap;
+        }
+        b->shape = fbct_batch_shape{64, 0, 16, 16};
     } catch (const std::bad_alloc &) {
         delete b;
         return fail(FBCT_ERR_OUT_OF_MEMORY, "batch tables");

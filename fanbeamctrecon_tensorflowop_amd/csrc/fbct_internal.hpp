@@ -171,6 +171,7 @@ struct BatchDevice {
     FpSliceDesc *fp_desc = nullptr;              // [n]
     FpaSliceDesc *fpa_desc = nullptr;            // [n]
     int *fp_prefix[kFpNumCfg] = {};              // [n+1] work items before slice s
+    int32_t *fp_order[kFpNumCfg] = {};           // [items] by decreasing tile rows
     int *fpa_prefix[kAdjNumTiles] = {};          // [n+1]
 };
 
@@ -221,6 +222,12 @@ struct fbct_batch {
     std::vector<int> fp_prefix[fbct::kFpNumCfg];
     std::vector<int> fpa_prefix[fbct::kAdjNumTiles];
     int fp_cap_max[fbct::kFpNumCfg] = {};
+    // FP items (prefix[s] + v * nb + b): first the fp_split items whose largest tile needs
+    // more than fp_small_cap rows, then the others (which run with that smaller tile), each
+    // class in item order so that a slice's items stay together
+    std::vector<int32_t> fp_order[fbct::kFpNumCfg];
+    int fp_split[fbct::kFpNumCfg] = {};
+    int fp_small_cap[fbct::kFpNumCfg] = {};
     int64_t fp_samples = 0, updates = 0;
     fbct_batch_shape shape{};
     std::mutex mu;

@@ -198,6 +198,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
             if (tid < kRowsPerPass * kPitch) {
                 const int aa = tid % kPitch;
                 const unsigned ga = (unsigned)min(max(alo + aa, 0), na - 1);
+#pragma unroll 2  // two taps' loads in flight per thread
                 for (int mm = tid / kPitch; mm < M; mm += kRowsPerPass)
                     stage(aa, mm, (unsigned)min(max(mlo + mm, 0), nm - 1) * (unsigned)A.nX + ga);
             }
@@ -207,6 +208,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
                 const int mm = m0 + (tid & 63);
                 if (mm >= M) continue;
                 const unsigned gm = (unsigned)min(max(mlo + mm, 0), nm - 1);
+#pragma unroll 2
                 for (int aa = wave; aa < kPitch; aa += kFpThreads / 64)
                     stage(aa, mm, (unsigned)min(max(alo + aa, 0), na - 1) * (unsigned)A.nX + gm);
             }
@@ -279,16 +281,18 @@ __global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, const int32_t 
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, view,
-// bundle), items of slice s are [prefix[s], prefix[s+1]).  A fixed grid of workgroups
-// strides over all items (item = blockIdx.x + k * gridDim.x), so neighbouring items --
-// the bundles and views of one slice -- run at the same time on different CUs.
+// bundle), items of slice s are [prefix[s], prefix[s+1]).  A launch takes the items
+// order[0, count) (a class of the batch's items ordered by tile size, all fitting `cap`
+// rows); a fixed grid of workgroups strides over them (order[blockIdx.x + k gridDim.x]).
 template <int G, typename TI, typename TO>
 __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc *__restrict__ desc,
-                                                              const int *__restrict__ prefix, int n, int total,
+                                                              const int *__restrict__ prefix, int n,
+                                                              const int32_t *__restrict__ order, int count, int cap,
                                                               const TI *img, TO *sino, int *err)
 {
     constexpr int cfg = fp_cfg(G);
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int t = blockIdx.x; t < count; t += gridDim.x) {
+        const int item = order[t];
         int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.nb = d.nb[cfg];
         A.nSlices = 1;
         A.nslab = d.nslab[cfg];
-        A.cap = d.cap[cfg];
+        A.cap = min(d.cap[cfg], cap);  // the launch's tile; the items it is given fit it
         A.scale = d.scale;
         const int local = item - prefix[lo];
         fp_bundle<1, G, TI, TO>(A, local % A.nb, local / A.nb, 0, err);
@@ -448,12 +452,12 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
 
 template <int G, typename TI, typename TO>
 static void launch_fp_batch_t(dim3 grid, size_t lds, hipStream_t stream, const fbct_batch &b, const BatchDevice &bd,
-                              const void *img, void *sino, int *err)
+                              const int32_t *order, int count, int cap, const void *img, void *sino, int *err)
 {
     constexpr int cfg = fp_cfg(G);
     const int n = (int)b.plans.size();
     hipLaunchKernelGGL((fp_batch_kernel<G, TI, TO>), grid, dim3(kFpThreads), lds, stream, bd.fp_desc,
-                       bd.fp_prefix[cfg], n, b.fp_prefix[cfg][n], static_cast<const TI *>(img),
+                       bd.fp_prefix[cfg], n, order, count, cap, static_cast<const TI *>(img),
                        static_cast<TO *>(sino), err);
 }
 
@@ -467,23 +471,38 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
     int *err = nullptr;
     if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
     hipError_t e;
-    const size_t lds = (size_t)fp_pitch(cfg) * b.fp_cap_max[cfg] * sizeof(float);
-    if (lds > 64 * 1024)
-        return fail(FBCT_ERR_UNSUPPORTED, "fbct_batch_fp: slab tile of %zu bytes exceeds 64 KiB", lds);
-    const long long wgs = std::min<long long>(total, (long long)b.shape.fp_wgs_per_cu * bd.num_cu);
-    const dim3 grid((unsigned)wgs);
+    const size_t row_bytes = (size_t)fp_pitch(cfg) * sizeof(float);
+    if ((size_t)b.fp_cap_max[cfg] * row_bytes > 64 * 1024)
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_batch_fp: slab tile of %zu bytes exceeds 64 KiB",
+                    (size_t)b.fp_cap_max[cfg] * row_bytes);
+    // items that fit the tile of kFpWgsPerCu workgroups per CU run in a launch of their
+    // own (they follow the larger ones in the order); fp_wgs_per_cu = 0: one workgroup
+    // per item, else a grid of that many per CU striding over the items
+    const int split = b.fp_split[cfg], small_cap = b.fp_small_cap[cfg];
+    for (int part = 0; part < 2; ++part) {
+        const int item0 = part == 0 ? 0 : split;
+        const int count = part == 0 ? split : total - split;
+        if (count == 0) continue;
+        const int cap = part == 0 ? b.fp_cap_max[cfg] : small_cap;
+        const size_t lds = (size_t)cap * row_bytes;
+        const long long wgs = b.shape.fp_wgs_per_cu == 0
+                                  ? count
+                                  : std::min<long long>(count, (long long)b.shape.fp_wgs_per_cu * bd.num_cu);
+        const dim3 grid((unsigned)wgs);
+        const int32_t *order = bd.fp_order[cfg] + item0;
 #define FBCT_FPB(G)                                                                                       \
     do {                                                                                                  \
-        if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_batch_t<G, float, float>(grid, lds, stream, b, bd, img, sino, err); \
-        else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_batch_t<G, bf16_t, float>(grid, lds, stream, b, bd, img, sino, err); \
-        else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_batch_t<G, float, bf16_t>(grid, lds, stream, b, bd, img, sino, err); \
-        else launch_fp_batch_t<G, bf16_t, bf16_t>(grid, lds, stream, b, bd, img, sino, err);          \
+        if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_batch_t<G, float, float>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
+        else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_batch_t<G, bf16_t, float>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
+        else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_batch_t<G, float, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
+        else launch_fp_batch_t<G, bf16_t, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino, err);          \
     } while (0)
-    if (cfg == 0) FBCT_FPB(4);
-    else FBCT_FPB(8);
+        if (cfg == 0) FBCT_FPB(4);
+        else FBCT_FPB(8);
 #undef FBCT_FPB
-    e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_batch_kernel launch: %s", hipGetErrorString(e));
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_batch_kernel launch: %s", hipGetErrorString(e));
+    }
     return FBCT_OK;
 }
 

@@ -818,7 +818,9 @@ fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, 
     if (k == kAdjNumTiles) return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_batch_fp_adjoint: tile width %d", b.shape.adj_tile_x);
     const int total = b.fpa_prefix[k][n];
     if (total == 0) return FBCT_OK;
-    const long long wgs = std::min<long long>(total, (long long)b.shape.adj_wgs_per_cu * bd.num_cu);
+    const long long wgs = b.shape.adj_wgs_per_cu == 0
+                              ? total
+                              : std::min<long long>(total, (long long)b.shape.adj_wgs_per_cu * bd.num_cu);
     const dim3 grid((unsigned)wgs);
     const int *prefix = bd.fpa_prefix[k];
     switch (b.shape.adj_tile_x) {

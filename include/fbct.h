@@ -247,11 +247,13 @@ typedef struct fbct_batch fbct_batch;
 /* Launch shape of a batch's persistent kernels. */
 typedef struct fbct_batch_shape {
     int32_t fp_rays;        /* rays per FP workgroup: 64 or 32                       */
-    int32_t fp_wgs_per_cu;  /* FP workgroups per compute unit in the grid, 1..32      */
+    int32_t fp_wgs_per_cu;  /* FP workgroups per compute unit in the grid, 1..32;
+                               0: one workgroup per work item                       */
     int32_t adj_tile_x;     /* FP^T pixel tile width: 8, 16, 32 or 64 (height 256/w), for
                                slices whose FP samples reach beyond the cells [-1, n-1];
                                other slices take fixed 15 x 15-pixel cell tiles      */
-    int32_t adj_wgs_per_cu; /* FP^T workgroups per compute unit in the grid, 1..32    */
+    int32_t adj_wgs_per_cu; /* FP^T workgroups per compute unit in the grid, 1..32;
+                               0: one workgroup per work item                       */
 } fbct_batch_shape;
 
 fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch **out);

@@ -73,21 +73,22 @@ def test_batch_equals_single_plans(uneven):
     _say("batch == single plans for %d slices" % len(plans))
 
 
+@pytest.mark.parametrize("wgs", [3, 0])  # 0: one workgroup per work item
 @pytest.mark.parametrize("rays", FP_RAYS)
 @pytest.mark.parametrize("tile", ADJ_TILE_X)
-def test_batch_shapes_agree(uneven, rays, tile):
+def test_batch_shapes_agree(uneven, rays, tile, wgs):
     scans, phantoms, plans, batch = uneven
     x = batch.pack_images(phantoms)
     batch.set_shape(fp_rays=64, fp_wgs_per_cu=8, adj_tile_x=16, adj_wgs_per_cu=8)
     y0 = batch.fp(x)
     g0 = batch.fp_adjoint(y0)
-    batch.set_shape(fp_rays=rays, fp_wgs_per_cu=3, adj_tile_x=tile, adj_wgs_per_cu=5)
+    batch.set_shape(fp_rays=rays, fp_wgs_per_cu=wgs, adj_tile_x=tile, adj_wgs_per_cu=wgs + 2 if wgs else 0)
     y1 = batch.fp(x)
     g1 = batch.fp_adjoint(y0)
     torch.cuda.synchronize()
     ey = rel_l2(y1.double().cpu().numpy(), y0.double().cpu().numpy())
     eg = rel_l2(g1.double().cpu().numpy(), g0.double().cpu().numpy())
-    _say("shape rays=%d tile=%d: fp %.2e adj %.2e" % (rays, tile, ey, eg))
+    _say("shape rays=%d tile=%d wgs=%d: fp %.2e adj %.2e" % (rays, tile, wgs, ey, eg))
     assert ey <= 1e-6 and eg <= 1e-6
     batch.set_shape(fp_rays=64, fp_wgs_per_cu=8, adj_tile_x=16, adj_wgs_per_cu=8)
 
