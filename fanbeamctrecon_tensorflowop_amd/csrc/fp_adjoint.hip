@@ -406,6 +406,19 @@ __device__ __forceinline__ void cell_weights(float4 g, float2 inv, float N, floa
     w[3] = sam;
 }
 
+// True when cell (ci, cj) can hold samples: every sample lies in the FOV disk of radius
+// r about texel coordinate (r - 1/2, r - 1/2) (its chord endpoints are on the circle), so
+// a cell whose nearest point is farther away (plus 0.01 texel for FP32 positions)
+// receives none and its views can be skipped.
+__device__ __forceinline__ bool cell_in_fov(const FpaArgs &A, float fci, float fcj)
+{
+    const float c0 = A.r - 0.5f;
+    const float ex = fmaxf(fmaxf(fci - c0, c0 - (fci + 1.0f)), 0.0f);
+    const float ey = fmaxf(fmaxf(fcj - c0, c0 - (fcj + 1.0f)), 0.0f);
+    const float rr = A.r + 0.01f;
+    return fmaf(ex, ex, ey * ey) <= rr * rr;
+}
+
 // Adds the S-slice contribution of one ray to the cell's corner partials.
 template <int S>
 __device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, const float (&sv)[S], float fci,
@@ -487,7 +500,7 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[q][s] = 0.0f;
 
-    if (ci <= A.nX - 1 && cj <= A.nY - 1) {
+    if (ci <= A.nX - 1 && cj <= A.nY - 1 && cell_in_fov(A, fci, fcj)) {
         for (int v = 0; v < A.nViews; ++v) {
             int clo, chi;
             candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
@@ -545,7 +558,8 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     CellStage<S> &st = sh.stage;
     const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
-    const bool active = ci <= A.nX - 1 && cj <= A.nY - 1;  // no early exit: every thread stages and waits
+    // no early exit: every thread stages and waits
+    const bool active = ci <= A.nX - 1 && cj <= A.nY - 1 && cell_in_fov(A, (float)ci, (float)cj);
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const float fci = (float)ci, fcj = (float)cj;
