@@ -193,7 +193,7 @@ fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch 
         // kFpWgsPerCu workgroups per CU first, then the rest, each in item order
         for (int k = 0; k < fbct::kFpNumCfg; ++k) {
             const int total = b->fp_prefix[k][n_slices];
-            const size_t row_bytes = static_cast<size_t>(fbct::kFpCfgSlab[k] + 1) * sizeof(float);
+            const size_t row_bytes = static_cast<size_t>(fbct::fp_pitch(k)) * sizeof(float);
             const int small_cap = static_cast<int>(std::min<size_t>(
                 static_cast<size_t>(b->fp_cap_max[k]), (160 * 1024 / fbct::kFpWgsPerCu) / row_bytes));
             std::vector<uint8_t> big(static_cast<size_t>(total), 0);

@@ -24,6 +24,9 @@ constexpr int kFpWgsPerCu = 7;
 constexpr int kFpNumCfg = 2;
 constexpr int kFpCfgRays[kFpNumCfg] = {64, 32};
 constexpr int kFpCfgSlab[kFpNumCfg] = {32, 24};
+// LDS tile row stride in taps: the slab's kFpCfgSlab + 1 taps and padding (bank spread)
+constexpr int kFpCfgPad[kFpNumCfg] = {0, 0};
+constexpr int fp_pitch(int cfg) { return kFpCfgSlab[cfg] + 1 + kFpCfgPad[cfg]; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each with its own
 // L2: the workgroup with linear id b runs on XCD b % 8.  xcd_block maps b to a logical

@@ -34,7 +34,7 @@ namespace {
 // Bundle configuration of a lane count G (kFpCfgRays[cfg] = kFpThreads / G) and its LDS
 // row pitch: taps [alo, alo + slab] of a slab.
 constexpr int fp_cfg(int G) { return kFpThreads / G == kFpCfgRays[0] ? 0 : 1; }
-constexpr int fp_pitch(int cfg) { return kFpCfgSlab[cfg] + 1; }
+
 
 struct FpArgs {
     const void *img;   // TI[n][nY][nX]
@@ -106,7 +106,8 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 {
     using TapT = typename Taps<S>::T;
     constexpr int kSlab = kFpCfgSlab[fp_cfg(G)];
-    constexpr int kPitch = kSlab + 1;
+    constexpr int kTaps = kSlab + 1;         // staged taps per row: [alo, alo + kSlab]
+    constexpr int kPitch = fp_pitch(fp_cfg(G));  // row stride of the tile, in taps
     float *tile = fp_lds;
 
     const int tid = threadIdx.x;
@@ -193,13 +194,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
             for (int s = 0; s < S; ++s) vp[s] = load_f_at(img[s], go * (unsigned)sizeof(TI));
             reinterpret_cast<TapT *>(tile)[mm * kPitch + aa] = val;
         };
-        if (xmajor) {  // a = i: kRowsPerPass rows of kPitch taps per pass of the workgroup
-            constexpr int kRowsPerPass = kFpThreads / kPitch;
-            if (tid < kRowsPerPass * kPitch) {
-                const int aa = tid % kPitch;
+        if (xmajor) {  // a = i: kRowsPerPass rows of kTaps taps per pass of the workgroup
+            constexpr int kRowsPerPass = kFpThreads / kTaps;
+            if (tid < kRowsPerPass * kTaps) {
+                const int aa = tid % kTaps;
                 const unsigned ga = (unsigned)min(max(alo + aa, 0), na - 1);
 #pragma unroll 2  // two taps' loads in flight per thread
-                for (int mm = tid / kPitch; mm < M; mm += kRowsPerPass)
+                for (int mm = tid / kTaps; mm < M; mm += kRowsPerPass)
                     stage(aa, mm, (unsigned)min(max(mlo + mm, 0), nm - 1) * (unsigned)A.nX + ga);
             }
         } else {  // a = j: 64 minor taps per wave, one major row per wave and pass
@@ -209,7 +210,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
                 if (mm >= M) continue;
                 const unsigned gm = (unsigned)min(max(mlo + mm, 0), nm - 1);
 #pragma unroll 2
-                for (int aa = wave; aa < kPitch; aa += kFpThreads / 64)
+                for (int aa = wave; aa < kTaps; aa += kFpThreads / 64)
                     stage(aa, mm, (unsigned)min(max(alo + aa, 0), na - 1) * (unsigned)A.nX + gm);
             }
         }
