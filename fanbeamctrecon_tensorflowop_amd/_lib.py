@@ -103,6 +103,7 @@ SIGNATURES = [
     ("fbct_fp", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fp_adjoint", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fp_ex", _I, [_P, _P, _I, _P, _I, _I, _P]),
+    ("fbct_fp_residual", _I, [_P, _P, _I, _P, _P, _I, _I, _P]),
     ("fbct_fp_adjoint_ex", _I, [_P, _P, _I, _P, _I, _I, _P]),
     ("fbct_bp", _I, [_P, _P, _P, _I, _P]),
     ("fbct_bp_adjoint", _I, [_P, _P, _P, _I, _P]),

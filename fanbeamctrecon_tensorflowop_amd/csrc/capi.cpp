@@ -53,6 +53,21 @@ fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slic
     return fbct::launch_fp(*plan, *t, image, FBCT_F32, sino, FBCT_F32, n_slices, static_cast<hipStream_t>(stream));
 }
 
+fbct_status fbct_fp_residual(fbct_plan *plan, const void *image, int image_dtype, const float *target,
+                             void *residual, int residual_dtype, int n_slices, void *stream)
+{
+    fbct_status s = check_common(plan, image, residual, n_slices, "fbct_fp_residual");
+    if (s != FBCT_OK || n_slices == 0) return s;
+    if (!target) return fbct::fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp_residual: null target");
+    if ((s = check_dtype(image_dtype, "fbct_fp_residual")) != FBCT_OK ||
+        (s = check_dtype(residual_dtype, "fbct_fp_residual")) != FBCT_OK)
+        return s;
+    const fbct::DeviceTables *t = nullptr;
+    if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
+    return fbct::launch_fp(*plan, *t, image, static_cast<fbct_dtype>(image_dtype), residual,
+                           static_cast<fbct_dtype>(residual_dtype), n_slices, static_cast<hipStream_t>(stream), target);
+}
+
 fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void *sino, int sino_dtype,
                        int n_slices, void *stream)
 {

@@ -281,8 +281,9 @@ fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **
 fbct_status device_cosine(fbct_plan *plan, const DeviceTables *t, const float **out);
 
 // Kernel launchers (fp.hip, bp.hip, filter.hip).  All asynchronous on `stream`.
+// target (optional, FP32 sinograms): the output is FP(img) - target.
 fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti,
-                      void *sino, fbct_dtype to, int n, hipStream_t stream);
+                      void *sino, fbct_dtype to, int n, hipStream_t stream, const float *target = nullptr);
 fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const void *sino,
                               fbct_dtype ti, void *img, fbct_dtype to, int n, hipStream_t stream);
 fbct_status launch_bp(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img,

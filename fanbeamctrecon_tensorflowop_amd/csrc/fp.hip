@@ -49,6 +49,7 @@ struct FpArgs {
     int nX, nY, nViews, nCols, nb, nSlices;
     int nslab, cap;
     float scale;
+    const float *target;  // optional FP32 [n][nViews][nCols]: store FP - target
 };
 
 // S interleaved slices of one tap: a native vector so one LDS read fetches all of them
@@ -142,6 +143,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     const int na = xmajor ? A.nX : A.nY;  // image extent along the major axis
     const int nm = xmajor ? A.nY : A.nX;
 
+    // residual mode: the targets are loaded now, so their latency hides behind the march
+    float tgt[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        tgt[s] = (A.target && lane == 0 && c < A.nCols)
+                     ? A.target[(size_t)min(s0 + s, A.nSlices - 1) * A.sino_stride + (size_t)v * A.nCols + c]
+                     : 0.0f;
     // FP32 sums within a slab, FP64 across slabs
     double accd[S];
 #pragma unroll
@@ -264,9 +272,11 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
     for (int s = 0; s < S; ++s) {
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
-        if (lane == 0 && c < A.nCols && s0 + s < A.nSlices)
-            store_f(static_cast<TO *>(A.sino), (size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c,
-                    (float)(accd[s] * (double)A.scale));
+        if (lane == 0 && c < A.nCols && s0 + s < A.nSlices) {
+            const size_t o = (size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c;
+            const float p = (float)(accd[s] * (double)A.scale);
+            store_f(static_cast<TO *>(A.sino), o, A.target ? p - tgt[s] : p);
+        }
     }
 }
 
@@ -302,6 +312,7 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         }
         const FpSliceDesc &d = desc[lo];
         FpArgs A;
+        A.target = nullptr;
         A.img = img + d.img_off;
         A.sino = sino + d.sino_off;
         A.ray_geo = d.ray_geo;
@@ -372,9 +383,10 @@ static fbct_status launch_fp_t(fbct_dtype ti, fbct_dtype to, dim3 grid, size_t l
 }
 
 fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img, fbct_dtype ti, void *sino,
-                      fbct_dtype to, int n, hipStream_t stream)
+                      fbct_dtype to, int n, hipStream_t stream, const float *target)
 {
     FpArgs A;
+    A.target = target;
     A.img = img;
     A.sino = sino;
     A.ray_geo = t.ray_geo;

@@ -174,6 +174,26 @@ class FanBeamPlan:
             return self._run("fbct_fp", x, self.sino_shape)
         return self._run_ex("fbct_fp_ex", x, self.sino_shape, out_dtype)
 
+    def fp_residual(self, image: torch.Tensor, target: torch.Tensor, out_dtype=torch.bfloat16) -> torch.Tensor:
+        """FP(image) - target in one pass (target: float32 sinograms), stored as out_dtype;
+        bitwise equal to ``(self.fp(image) - target).to(out_dtype)``."""
+        x = self._prep(image, self.image_shape, "fp_residual", _STORAGE)
+        y = self._prep(target, self.sino_shape, "fp_residual target")
+        if y.shape[0] != x.shape[0] or y.device != x.device:
+            raise ValueError("fp_residual: target must have the image batch's size and device")
+        lib = _lib.load()
+        n = x.shape[0]
+        out = torch.empty((n,) + tuple(self.sino_shape), dtype=out_dtype, device=x.device)
+        if n == 0:
+            return out
+        with torch.cuda.device(x.device):
+            stream = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
+            st = lib.fbct_fp_residual(self._h, ctypes.c_void_p(x.data_ptr()), _DTYPE[x.dtype],
+                                      ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                      _DTYPE[out_dtype], int(n), stream)
+        _lib.check(st, "fbct_fp_residual")
+        return out
+
     def fp_adjoint(self, sino: torch.Tensor, out_dtype=torch.float32) -> torch.Tensor:
         """Exact transpose of :meth:`fp`: sinogram -> image (float32 or bfloat16 storage)."""
         y = self._prep(sino, self.sino_shape, "fp_adjoint", _STORAGE)

@@ -152,6 +152,15 @@ fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slic
 fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void *sino, int sino_dtype,
                        int n_slices, void *stream);
 
+/* Residual of the forward projection: residual = FP(image) - target, the FP32 difference
+ * of the FP32 projection and target stored in residual_dtype (bitwise what fbct_fp_ex to
+ * FP32 followed by the subtraction and a conversion would store), in one pass.  target is
+ * an FP32 sinogram batch [n][nViews][nCols].  No reference counterpart: it fuses the
+ * data-fidelity residual of an iterative reconstruction (the reference callers subtract
+ * in the framework after the FP op). */
+fbct_status fbct_fp_residual(fbct_plan *plan, const void *image, int image_dtype, const float *target,
+                             void *residual, int residual_dtype, int n_slices, void *stream);
+
 /* Exact matched adjoint of fbct_fp (transpose of the same interpolation weights).
  * sino -> image.  No reference counterpart (the reference has no gradient).  The
  * sinogram must be finite: a NaN or infinity spreads to every pixel whose candidate

@@ -94,6 +94,10 @@ def test_execute_validation_without_gpu():
     assert lib.fbct_fbp(h, fake, fake, 1, null, 0, null) == 7
     assert lib.fbct_fbp_workspace_size(h, 2) == 2 * plan.n_views * plan.n_cols * 4
     assert lib.fbct_ramp_filter(h, fake, fake, 3, null) == 1  # in == out
+    # fused residual: a target is required, storage types are checked
+    assert lib.fbct_fp_residual(h, fake, 0, null, fake, 1, 0, null) == 0
+    assert lib.fbct_fp_residual(h, fake, 0, null, fake, 1, 2, null) == 1
+    assert lib.fbct_fp_residual(h, fake, 0, fake, fake, 5, 2, null) == 1
 
 
 def test_uneven_batch_host_side():
