@@ -377,16 +377,21 @@ constexpr int kCells = kCellTile * kCellTile;  // one per thread
 constexpr int kCombineS = 4;                   // slices combined per LDS pass
 constexpr float kCellRho = 0.70710678f;        // half a cell diagonal (texels)
 
-// Bilinear weight sums of the samples of ray g = (u0, w0, du, dw) (inv = (1/du, 1/dw), N
-// samples) that lie in cell (fci, fcj); w[] for corners (0,0) (1,0) (0,1) (1,1).
-__device__ __forceinline__ void cell_weights(float4 g, float2 inv, float N, float fci, float fcj, float w[4])
+// Closed-form sums over the samples of ray g = (u0, w0, du, dw) (inv = (1/du, 1/dw), N
+// samples, dzw = du dw) that lie in cell (fci, fcj): r = (n, sum of u-fractions, sum of
+// w-fractions, sum of their products).  The bilinear weights the cell's corners (0,0)
+// (1,0) (0,1) (1,1) receive are r0 - r1 - r2 + r3, r1 - r3, r2 - r3 and r3 (formed once
+// per cell from the accumulated sums in cell_combine).
+__device__ __forceinline__ void cell_sums(float4 g, float2 inv, float N, float dzw, float fci, float fcj, float r[4])
 {
     // sample range inside the cell, per axis, then intersected with [0, N): the samples
     // that precede boundary b along the ray number ceil((b - u0) / du) in either
     // direction (a sample exactly on b goes to the cell it leaves, where its bilinear
-    // weights are the same), so the range is [ceil(min q), ceil(max q))
-    const float qu0 = (fci - g.x) * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
-    const float qw0 = (fcj - g.y) * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
+    // weights are the same), so the range is [ceil(min q), ceil(max q)); the upper
+    // boundary uses the expression its other cell uses for its lower one
+    const float eu = fci - g.x, ew = fcj - g.y;
+    const float qu0 = eu * inv.x, qu1 = (fci + 1.0f - g.x) * inv.x;
+    const float qw0 = ew * inv.y, qw1 = (fcj + 1.0f - g.y) * inv.y;
     const float lou = ceilf(fminf(qu0, qu1)), hiu = ceilf(fmaxf(qu0, qu1));
     const float low = ceilf(fminf(qw0, qw1)), hiw = ceilf(fmaxf(qw0, qw1));
     const float klo = fminf(fmaxf(fmaxf(lou, low), 0.0f), N);
@@ -394,16 +399,13 @@ __device__ __forceinline__ void cell_weights(float4 g, float2 inv, float N, floa
     const float n = fmaxf(khi - klo, 0.0f);
     // fractions of the first sample: the cell index is subtracted before the step is
     // added, so the FP32 rounding is that of the distance, not of the texel coordinate
-    const float ta = fmaf(klo, g.z, g.x - fci), tm = fmaf(klo, g.w, g.y - fcj);
+    const float ta = fmaf(klo, g.z, -eu), tm = fmaf(klo, g.w, -ew);
     const float t1 = 0.5f * fmaf(n, n, -n);                    // sum of j < n
     const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);  // sum of j^2
-    const float sa = fmaf(g.z, t1, n * ta);                      // sum of u-fractions
-    const float sm = fmaf(g.w, t1, n * tm);                      // sum of w-fractions
-    const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, g.w, tm * g.z), t1, g.z * g.w * t2));
-    w[0] = (n - sa) - (sm - sam);
-    w[1] = sa - sam;
-    w[2] = sm - sam;
-    w[3] = sam;
+    r[0] = n;
+    r[1] = fmaf(g.z, t1, n * ta);  // sum of u-fractions
+    r[2] = fmaf(g.w, t1, n * tm);  // sum of w-fractions
+    r[3] = fmaf(n, ta * tm, fmaf(fmaf(ta, g.w, tm * g.z), t1, dzw * t2));
 }
 
 // True when cell (ci, cj) can hold samples: every sample lies in the FOV disk of radius
@@ -421,18 +423,19 @@ __device__ __forceinline__ bool cell_in_fov(const FpaArgs &A, float fci, float f
 
 // Adds the S-slice contribution of one ray to the cell's corner partials.
 template <int S>
-__device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, const float (&sv)[S], float fci,
-                                                float fcj, float (&acc)[4][S])
+__device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, float dzw, const float (&sv)[S],
+                                                float fci, float fcj, float (&acc)[4][S])
 {
-    float w[4];
-    cell_weights(g, inv, N, fci, fcj, w);
+    float r[4];
+    cell_sums(g, inv, N, dzw, fci, fcj, r);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int s = 0; s < S; ++s) acc[q][s] = fmaf(w[q], sv[s], acc[q][s]);
+        for (int s = 0; s < S; ++s) acc[q][s] = fmaf(r[q], sv[s], acc[q][s]);
 }
 
-// Combines the corner partials acc[4][S] of the tile's cells into its 15 x 15 pixels of
+// Combines the cell sums acc[4][S] (n, Su, Sw, Suw, each times the ray values) of the
+// tile's cells, as corner partials, into its 15 x 15 pixels of
 // slices [s0, s0 + S), kCombineS slices per pass through part (4 x kCombineS x kCells
 // floats of LDS, free when called; every thread calls it).  Pixel (px, py) collects
 // corner dx of cell px - dx (dx = 0, 1), and, on the borders, the clamped outer corner
@@ -458,9 +461,14 @@ __device__ __forceinline__ void cell_combine(const FpaArgs &A, float *part, cons
     for (int sb = 0; sb < S; sb += kCombineS) {
         constexpr int kC = S < kCombineS ? S : kCombineS;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int k = 0; k < kC; ++k) part[(q * kC + k) * kCells + threadIdx.x] = acc[q][sb + k];
+        for (int k = 0; k < kC; ++k) {  // corner weights from the sums (n, Su, Sw, Suw)
+            const float sn = acc[0][sb + k], su = acc[1][sb + k], sw = acc[2][sb + k], suw = acc[3][sb + k];
+            const float c01 = sw - suw;
+            part[(0 * kC + k) * kCells + threadIdx.x] = (sn - su) - c01;
+            part[(1 * kC + k) * kCells + threadIdx.x] = su - suw;
+            part[(2 * kC + k) * kCells + threadIdx.x] = c01;
+            part[(3 * kC + k) * kCells + threadIdx.x] = suw;
+        }
         __syncthreads();
         if (out) {
 #pragma unroll
@@ -513,7 +521,7 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
                 float sv[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
-                cell_accumulate<S>(g, inv, N, sv, fci, fcj, acc);
+                cell_accumulate<S>(g, inv, N, g.z * g.w, sv, fci, fcj, acc);
             }
         }
     }
@@ -537,8 +545,8 @@ template <int S>
 struct CellStage {
     static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
     float val[kStageVB][kStageCols][kPitch];
-    // ray tables of the staged columns: (u0, w0, du, dw) and (1/du, 1/dw, N, -), each a
-    // 16-byte record so that 16 lanes reading distinct columns hit distinct banks
+    // ray tables of the staged columns: (u0, w0, du, dw) and (1/du, 1/dw, N, du dw), each
+    // a 16-byte record so that 16 lanes reading distinct columns hit distinct banks
     float4 geo[kStageVB][kStageCols];
     float4 aux[kStageVB][kStageCols];
 };
@@ -589,15 +597,16 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     // cell_accumulate
     auto global_ray = [&](int v, int c) {
         const size_t rb = (size_t)v * A.nCols + c;
-        float w[4];
-        cell_weights(A.ray_geo[rb], A.ray_inv[rb], (float)A.ray_n[rb], fci, fcj, w);
+        float r[4];
+        const float4 g = A.ray_geo[rb];
+        cell_sums(g, A.ray_inv[rb], (float)A.ray_n[rb], g.z * g.w, fci, fcj, r);
         unsigned off = (unsigned)rb * (unsigned)sizeof(TI);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const float val = load_f_at(gbase, off);
             if (s < last) off += sbytes;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q][s] = fmaf(w[q], val, acc[q][s]);
+            for (int q = 0; q < 4; ++q) acc[q][s] = fmaf(r[q], val, acc[q][s]);
             __builtin_amdgcn_sched_barrier(0);
         }
     };
@@ -638,7 +647,8 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
                 st.geo[m][lane] = A.ray_geo[rb];
             } else if (wave == 1) {
                 const float2 iv = A.ray_inv[rb];
-                st.aux[m][lane] = make_float4(iv.x, iv.y, (float)A.ray_n[rb], 0.0f);
+                const float4 g = A.ray_geo[rb];
+                st.aux[m][lane] = make_float4(iv.x, iv.y, (float)A.ray_n[rb], g.z * g.w);
             }
         }
         __syncthreads();
@@ -661,7 +671,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
                     const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
                     sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
                 }
-                cell_accumulate<S>(st.geo[m][lc], make_float2(ax.x, ax.y), ax.z, sv, fci, fcj, acc);
+                cell_accumulate<S>(st.geo[m][lc], make_float2(ax.x, ax.y), ax.z, ax.w, sv, fci, fcj, acc);
             }
             for (; c <= chi; ++c) global_ray(v, c);
         }
