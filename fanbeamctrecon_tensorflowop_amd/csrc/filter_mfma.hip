@@ -69,8 +69,10 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-    const long long r0 = (long long)blockIdx.x * kBM;
-    const int c0 = blockIdx.y * kBN;
+    // each XCD takes whole row tiles, their column tiles consecutive (P rows stay in its L2)
+    const unsigned lblk = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const long long r0 = (long long)(lblk / gridDim.y) * kBM;
+    const int c0 = (int)(lblk % gridDim.y) * kBN;
     // staging assignment: thread -> (row or column tid / 2, 16 k-values at kh)
     const int srow = tid >> 1, kh = (tid & 1) * 16;
     const long long arow = r0 + srow;
@@ -236,7 +238,8 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     if (H.Np % kBN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC)
         return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: operand planes not padded to the tile");
     const long long gx = (rows + kBM - 1) / kBM;
-    if (gx > 0x7fffffffLL) return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
+    if (gx > 0x7fffffffLL || gx * ((H.nC + kBN - 1) / kBN) > 0xffffffffLL)
+        return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
     dim3 grid((unsigned)gx, (H.nC + kBN - 1) / kBN);
     if (ti == FBCT_BF16) {
         if (w) launch_gemm_p<uint16_t, true>(grid, stream, A);
