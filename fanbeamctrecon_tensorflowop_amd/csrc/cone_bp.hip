@@ -166,7 +166,10 @@ constexpr int kLdsPitch = kLdsCols + 1;        // floats between the views of a 
 template <int VB>
 constexpr int lds_stride()
 {
-    return (VB * kLdsPitch) | 1;
+    // odd, and 10 beyond the packed VB * kLdsPitch: 141 for two views (strides 131 to 141
+    // measured: 141 has the fewest bank conflicts between the lanes' rows; up to 142 keeps
+    // four workgroups of 72 staged rows per CU)
+    return ((VB * kLdsPitch) | 1) + 10;
 }
 // Staged rows per view for z-chunks of KZ voxels: a 16 x 16 x KZ tile's taps span at most
 // 42 (KZ = 16) or 71 (KZ = 32) detector rows at the 0.7 mm / 1.0 mm, SID 541, SDD 949
