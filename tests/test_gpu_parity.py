@@ -191,9 +191,10 @@ def test_batch_equals_per_slice(small, n):
         assert rel_l2(ba[s], host(plan.fp_adjoint(dev(y[s:s + 1])))[0]) <= 1e-6
 
 
-@pytest.mark.parametrize("n", [4, 8, 12, 16, 19])
+@pytest.mark.parametrize("n", [2, 3, 4, 8, 12, 16, 19])
 def test_clinical_batch_equals_per_slice(clinical, n):
-    # the clinical grid selects the shared-slice kernels: FP with 4-slice tiles, the
+    # the clinical grid selects the shared-slice kernels: FP with 2-slice tiles (n = 2, 3:
+    # two tile classes, 88 of the 128 rows) or 4-slice tiles, the
     # adjoint with 16-, 8- or 4-slice groups staged through LDS (one slice: the
     # cell-driven kernel); a ragged last group is padded.  The adjoint runs on smooth
     # sinograms (the slices' projections): the gather and cell-driven kernels round the
