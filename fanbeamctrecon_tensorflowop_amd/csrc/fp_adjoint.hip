@@ -1,4 +1,5 @@
-// fp_adjoint.hip -- matched adjoint of the ray-driven FP, gathered per pixel (gfx950).
+// fp_adjoint.hip -- matched adjoint of the ray-driven FP (gfx950): cell-driven where
+// every sample lies in the image's texel cells [-1, n-1], else gathered per pixel.
 //
 // The FP (fp.hip; reference FP/transform.cu:103-146) forms
 //     p[v][c] = 0.125 dx * sum_k  sum_taps  w_tap(u_k, w_k) * img[clamp(tap)]
@@ -9,8 +10,10 @@
 // clamped taps of a sample give pixel i (the clamp folds the taps -1 and nX, and any
 // sample beyond the image edge, into the border pixels).
 //
-// Each thread owns one output pixel and accumulates over all views, so every pixel is
-// written exactly once and no atomics are needed.  Per view, the candidate rays are
+// Both forms write every pixel exactly once, without atomics.  The cell-driven kernels
+// (section "cell-driven variant" below) sum each ray's samples per texel cell in closed
+// form.  The per-pixel gather: each thread owns one output pixel and accumulates over
+// all views.  Per view, the candidate rays are
 // the columns whose lines pass within the support of the pixel (radius rho around its
 // centre, rho = sqrt(2) inside the image), found from the pixel's fan angle; per
 // candidate ray the sample range whose positions fall in the support is computed in
@@ -362,12 +365,12 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_kernel(FpaArgs A)
 // view and candidate ray (lines within half a cell diagonal of the cell centre) it finds
 // the ray's samples inside the cell: along each axis the samples before boundary b number
 // ceil((b - u0) / du) (du > 0; floor + 1 for du < 0), evaluated by the same expression in
-// both cells that share b, so the cells partition every ray's samples exactly.  The four
-// bilinear weight sums of those n samples (fractions ta + j du, tm + j dw) are
-// closed-form, as in the forward kernel's cell walk, and accumulate (FP32) into four
-// corner partials per slice.  Corner partials are combined per pixel through LDS at the
-// end, clamp-to-edge folding the outer corners of cells -1 and n-1 into the border
-// pixels.  Valid while every sample lies in the cells [-1, n-1] of both axes
+// both cells that share b, so the cells partition every ray's samples exactly.  The sums
+// over those n samples of 1, the u- and w-fractions and their products (fractions
+// ta + j du, tm + j dw) are closed-form, as in the forward kernel's cell walk, and
+// accumulate (FP32) times the ray's value per slice; at the end they give the four corner
+// weights, which are combined per pixel through LDS, clamp-to-edge folding the outer
+// corners of cells -1 and n-1 into the border pixels.  Valid while every sample lies in the cells [-1, n-1] of both axes
 // (2 rFOV/dx - n + 1/2 < 1, square images); otherwise the gather kernels run.  Every
 // variant below evaluates a slice with the same arithmetic in the same order, so results
 // do not depend on how many slices share a launch.
@@ -421,7 +424,7 @@ __device__ __forceinline__ bool cell_in_fov(const FpaArgs &A, float fci, float f
     return fmaf(ex, ex, ey * ey) <= rr * rr;
 }
 
-// Adds the S-slice contribution of one ray to the cell's corner partials.
+// Adds the S-slice contribution of one ray to the cell's sums.
 template <int S>
 __device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, float dzw, const float (&sv)[S],
                                                 float fci, float fcj, float (&acc)[4][S])
@@ -676,7 +679,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             for (; c <= chi; ++c) global_ray(v, c);
         }
     }
-    __syncthreads();  // staging reads are done: the union turns into corner partials
+    __syncthreads();  // staging reads are done: the union turns into corner weights
     cell_combine<S, TO>(A, sh.part, acc, tx, ty, s0);
 }
 
