@@ -1,4 +1,5 @@
-// fp.hip -- ray-driven fan-beam forward projection and its matched adjoint (gfx950).
+// fp.hip -- ray-driven fan-beam forward projection (gfx950); its matched adjoint is in
+// fp_adjoint.hip.
 //
 // Reference: GenerateRayTraceMap_FB_ReconImage_OneView_kernel (FP/transform.cu:64-101)
 // builds per-ray start/step/N; RayTrace_ReconImage_OneView_kernel (:103-146) sums
@@ -10,12 +11,15 @@
 //     slab by slab along the axis the bundle is closest to;  for every slab it stages
 //     the exact bounding box of the taps its samples touch into LDS (clamped to the
 //     image edge, which reproduces the clamp-to-edge texture address mode);
-//   * G lanes share a ray, each takes a contiguous share of the ray's samples in the
-//     slab and walks it texel cell by texel cell: the bilinear weights of a cell's
-//     samples are summed in closed form and its four taps are read once (a ray spends
-//     6-8 samples in a cell at the 1/8-pixel step);
+//   * G lanes share a ray, each takes the ray's samples in a fixed share of the slab's
+//     major coordinate (ending on texel boundaries) and walks it texel cell by texel
+//     cell: the bilinear weights of a cell's samples are summed in closed form and its
+//     four taps are read once (a ray spends 6-8 samples in a cell at the 1/8-pixel step);
+//   * the plan orders the (view, bundle) work items by the size of their largest tile;
+//     the items whose tile fits the LDS share of kFpWgsPerCu workgroups per CU run in a
+//     launch with that smaller tile, the others in a launch with the full one;
 //   * FP32 partial sums per slab, FP64 across slabs; one cross-lane reduction per ray at
-//     the end and one coalesced store.
+//     the end and one coalesced store (optionally minus a target: the FP residual).
 // A sample belongs to the slab containing its major coordinate, decided from the same
 // FP32 position the interpolation uses, so every sample is counted exactly once.
 
