@@ -106,6 +106,22 @@ __device__ __forceinline__ int interp_pos(float xi, int nCols, float &al)
     return (int)fc;
 }
 
+// Weights of the taps c0 and c0 + 1 of one view: (1 - al, al), times the fan-beam
+// distance weight 1 / (U^2 + t^2) for the weighted (FBP) form, so that a slice's update
+// is two FMAs.
+template <bool kWeighted>
+__device__ __forceinline__ void bp_tap_weights(float al, float U, float t, float &w0, float &w1)
+{
+    if constexpr (kWeighted) {
+        const float w = __builtin_amdgcn_rcpf(fmaf(U, U, t * t));
+        w1 = al * w;
+        w0 = w - w1;
+    } else {
+        w1 = al;
+        w0 = 1.0f - al;
+    }
+}
+
 // Slices [slice0 + S*blockIdx.z, +S) of the batch.
 template <bool kWeighted, int S, int kNc>
 __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
@@ -130,13 +146,13 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
             const int c0 = interp_pos(xi, A.nCols, al);
             const int c1 = min(c0 + 1, c1max);
             const float *P = sino + (size_t)v * A.nCols;
-            const float w = kWeighted ? __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : 1.0f;
+            float w0, w1;
+            bp_tap_weights<kWeighted>(al, U, t, w0, w1);
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const float p0 = P[(size_t)s * A.sino_stride + c0];
                 const float p1 = P[(size_t)s * A.sino_stride + c1];
-                const float val = fmaf(al, p1 - p0, p0);
-                acc[s] = kWeighted ? fmaf(val, w, acc[s]) : acc[s] + val;
+                acc[s] = fmaf(w1, p1, fmaf(w0, p0, acc[s]));
             }
         }
     }
@@ -228,7 +244,8 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
             if (!column_of<kNc>(A, pg, v, xi, U, tt)) continue;
             float al;
             const int c0 = interp_pos(xi, A.nCols, al);
-            const float w = kWeighted ? __builtin_amdgcn_rcpf(fmaf(U, U, tt * tt)) : 1.0f;
+            float w0, w1;
+            bp_tap_weights<kWeighted>(al, U, tt, w0, w1);
             const int lc = c0 - c_lo[m];
             float p0[S], p1[S];
             if ((unsigned)lc < (unsigned)(kBpStageCols - 1) && c0 + 1 < A.nCols) {
@@ -249,10 +266,7 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
                 }
             }
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const float val = fmaf(al, p1[s] - p0[s], p0[s]);
-                acc[s] = kWeighted ? fmaf(val, w, acc[s]) : acc[s] + val;
-            }
+            for (int s = 0; s < S; ++s) acc[s] = fmaf(w1, p1[s], fmaf(w0, p0[s], acc[s]));
         }
     }
     if (!valid) return;
