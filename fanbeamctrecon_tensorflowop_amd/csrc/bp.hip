@@ -185,11 +185,14 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
     unsigned t, g;  // XCD-aware (tile, slice group)
     xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
     const int tx0 = (int)(t % gridDim.x) * kBpTx, ty0 = (int)(t / gridDim.x) * kBpTy;
-    const int i = tx0 + (int)threadIdx.x % kBpTx;
-    const int j = ty0 + (int)threadIdx.x / kBpTx;
-    const size_t sl = (size_t)slice0 + (size_t)S * g;
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    // wave w takes tile rows 4w..4w+3; each 16 lanes of it a 4x4 pixel block, whose
+    // detector columns lie close together (fewer LDS bank conflicts than a 16-pixel row)
+    static_assert(kBpTx == 16 && kBpTy == 16, "lane layout assumes 16x16 tiles");
+    const int i = tx0 + 4 * (lane >> 4) + (lane & 3);
+    const int j = ty0 + 4 * wave + ((lane >> 2) & 3);
+    const size_t sl = (size_t)slice0 + (size_t)S * g;
     // no early exit: every thread stages and waits
     const bool valid = i < A.nX && j < A.nY;
     const PixelGeo pg = pixel_geo(A, min(i, A.nX - 1), min(j, A.nY - 1));
