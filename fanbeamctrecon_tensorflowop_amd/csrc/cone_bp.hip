@@ -205,8 +205,11 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
     unsigned t, gz;
     xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, gz);
     const int tx0 = (int)(t % gridDim.x) * kTx, ty0 = (int)(t / gridDim.x) * kTy;
-    const int i = tx0 + (int)threadIdx.x % kTx;
-    const int j = ty0 + (int)threadIdx.x / kTx;
+    // each wave takes an 8x8 pixel quarter of the tile: its taps spread over fewer staged
+    // rows and columns than a 16x4 strip's (fewer LDS bank conflicts)
+    static_assert(kTx == 16 && kTy == 16, "lane layout assumes 16x16 tiles");
+    const int i = tx0 + 8 * (((int)threadIdx.x >> 6) & 1) + ((int)threadIdx.x & 7);
+    const int j = ty0 + 8 * ((int)threadIdx.x >> 7) + (((int)threadIdx.x >> 3) & 7);
     const int zc = gz % A.nzc;
     const size_t vi = gz / A.nzc;
     const int k0 = zc * KZ;
