@@ -437,6 +437,17 @@ __device__ __forceinline__ void cell_accumulate(float4 g, float2 inv, float N, f
         for (int s = 0; s < S; ++s) acc[q][s] = fmaf(r[q], sv[s], acc[q][s]);
 }
 
+// Cell (lx, ly) of this thread in the 16 x 16 tile: wave w takes cell rows 4w..4w+3 and
+// each 16 lanes of it a 4 x 4 block, whose candidate columns lie close together (fewer
+// LDS bank conflicts on the staged columns than a 16-cell row).
+__device__ __forceinline__ void tile_cell(int &lx, int &ly)
+{
+    static_assert(kCellTile == 16 && kCells == 256, "lane layout assumes 16 x 16 cells, 256 threads");
+    const int t = (int)threadIdx.x, lane = t & 63;
+    lx = 4 * (lane >> 4) + (lane & 3);
+    ly = 4 * (t >> 6) + ((lane >> 2) & 3);
+}
+
 // Combines the cell sums acc[4][S] (n, Su, Sw, Suw, each times the ray values) of the
 // tile's cells, as corner partials, into its 15 x 15 pixels of
 // slices [s0, s0 + S), kCombineS slices per pass through part (4 x kCombineS x kCells
@@ -447,7 +458,9 @@ template <int S, typename TO>
 __device__ __forceinline__ void cell_combine(const FpaArgs &A, float *part, const float (&acc)[4][S], int tx, int ty,
                                              int s0)
 {
-    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    int lx, ly;
+    tile_cell(lx, ly);
+    const int cid = ly * kCellTile + lx;
     const int px = tx * kCellOut + lx, py = ty * kCellOut + ly;
     const bool out = lx < kCellOut && ly < kCellOut && px < A.nX && py < A.nY;
     int xc[3], xd[3], yc[3], yd[3];  // local cell column and corner of each x (y) term
@@ -467,10 +480,10 @@ __device__ __forceinline__ void cell_combine(const FpaArgs &A, float *part, cons
         for (int k = 0; k < kC; ++k) {  // corner weights from the sums (n, Su, Sw, Suw)
             const float sn = acc[0][sb + k], su = acc[1][sb + k], sw = acc[2][sb + k], suw = acc[3][sb + k];
             const float c01 = sw - suw;
-            part[(0 * kC + k) * kCells + threadIdx.x] = (sn - su) - c01;
-            part[(1 * kC + k) * kCells + threadIdx.x] = su - suw;
-            part[(2 * kC + k) * kCells + threadIdx.x] = c01;
-            part[(3 * kC + k) * kCells + threadIdx.x] = suw;
+            part[(0 * kC + k) * kCells + cid] = (sn - su) - c01;
+            part[(1 * kC + k) * kCells + cid] = su - suw;
+            part[(2 * kC + k) * kCells + cid] = c01;
+            part[(3 * kC + k) * kCells + cid] = suw;
         }
         __syncthreads();
         if (out) {
@@ -495,7 +508,8 @@ __device__ __forceinline__ void cell_combine(const FpaArgs &A, float *part, cons
 template <int S, typename TI, typename TO>
 __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int s0, float *part)
 {
-    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    int lx, ly;
+    tile_cell(lx, ly);
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
     const float fci = (float)ci, fcj = (float)cj;
     // cell centre in mm: texel coordinate ci + 1/2 (texel u sits at (u + 1/2 - r) dx)
@@ -567,7 +581,8 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
     constexpr int VB = kStageVB;
     constexpr int SQ = S / 4;  // slices staged per wave
     CellStage<S> &st = sh.stage;
-    const int lx = (int)threadIdx.x % kCellTile, ly = (int)threadIdx.x / kCellTile;
+    int lx, ly;
+    tile_cell(lx, ly);
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
     // no early exit: every thread stages and waits
     const bool active = ci <= A.nX - 1 && cj <= A.nY - 1 && cell_in_fov(A, (float)ci, (float)cj);
