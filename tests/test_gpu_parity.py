@@ -222,6 +222,29 @@ def test_clinical_bp_groups_equal_per_slice(clinical, n):
         assert rel_l2(bf[s], host(plan.fbp(dev(y[s:s + 1])))[0]) <= 1e-6
 
 
+def test_staged_groups_on_ragged_tiles(oracle_mod):
+    # 500 x 468 pixels (neither side a multiple of the 16-pixel tile) and 48 slices: BP and
+    # FBP run three 16-slice groups that stage their detector columns in LDS, with edge
+    # tiles partly outside the image; each slice equals its single-slice call (the
+    # unstaged kernel) and the first, a phantom's projections (the oracle tolerance is
+    # stated for phantoms), matches the oracle
+    p, a = workloads.scan(nx=500, ny=468, rfov=250.0, sid=1000.0, sdd=2000.0, n_cols=256, dcol_rad=0.0022,
+                          n_views=180)
+    plan = FanBeamPlan(p, a)
+    n = 48
+    y = np.random.default_rng(48).uniform(0, 1, size=(n,) + plan.sino_shape).astype(np.float32)
+    y[0] = host(plan.fp(dev(workloads.ellipse_phantom(500, 468, 250.0, 8, seed=4)[None])))[0]
+    bb, bf = host(plan.bp(dev(y))), host(plan.fbp(dev(y)))
+    for s in (0, n // 2, n - 1):
+        assert rel_l2(bb[s], host(plan.bp(dev(y[s:s + 1])))[0]) <= 1e-6
+        assert rel_l2(bf[s], host(plan.fbp(dev(y[s:s + 1])))[0]) <= 1e-6
+    o = oracle_mod.Oracle(p, a)
+    for name, got, ref in (("bp", bb[:1], o.bp(y[:1])), ("fbp", bf[:1], o.fbp(y[:1]))):
+        err = rel_l2(got, ref)
+        _say("ragged 16-slice groups %s rel_l2=%.3e" % (name, err))
+        assert err <= TOL, (name, err)
+
+
 def test_zero_slices():
     plan = FanBeamPlan(*workloads.tiny_scan())
     out = plan.fp(torch.zeros(0, 16, 16, device="cuda"))
