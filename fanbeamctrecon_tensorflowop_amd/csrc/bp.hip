@@ -34,9 +34,10 @@ struct BpArgs {
     float *img;
     const float2 *cs;
     const double2 *csd;
+    const int2 *fov_row;  // per row the FOV pixel range (FP64 membership of the centres)
     long long img_stride, sino_stride;
     int nX, nY, nViews, nCols;
-    float R, xmin, ymin, dx, dy, r2;
+    float R, xmin, ymin, dx, dy;
     float xi_scale, xi_offset;
     int edge_exact;
     double tan_min, tan_max;
@@ -136,7 +137,8 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_kernel(BpArgs A, int slice0)
     float acc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = 0.0f;
-    if (pg.x * pg.x + pg.y * pg.y <= A.r2) {
+    const int2 fr = A.fov_row[j];
+    if (i >= fr.x && i <= fr.y) {
         const float *sino = A.sino + sl * A.sino_stride;
         const int c1max = A.nCols - 1;
         for (int v = 0; v < A.nViews; ++v) {
@@ -196,7 +198,8 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
     // no early exit: every thread stages and waits
     const bool valid = i < A.nX && j < A.nY;
     const PixelGeo pg = pixel_geo(A, min(i, A.nX - 1), min(j, A.nY - 1));
-    const bool inside = valid && pg.x * pg.x + pg.y * pg.y <= A.r2;
+    const int2 fr = A.fov_row[min(j, A.nY - 1)];
+    const bool inside = valid && i >= fr.x && i <= fr.y;
     const PixelGeo pc = pixel_geo(A, min(tx0 + ((lane & 1) ? kBpTx - 1 : 0), A.nX - 1),
                                   min(ty0 + ((lane & 2) ? kBpTy - 1 : 0), A.nY - 1));
     const float *sino = A.sino + sl * A.sino_stride;  // slice group base
@@ -277,36 +280,138 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
     for (int s = 0; s < S; ++s) A.img[(sl + s) * A.img_stride + (size_t)j * A.nX + i] = acc[s];
 }
 
-// Transpose: a workgroup owns one detector row (view v, slice) in LDS, visits every
-// pixel of the FOV and scatters the two linear-interpolation weights (clamped taps
-// folded into the edge columns) with ds_add_f32; the row is stored once.
-template <bool kWeighted, int kNc>
-__global__ __launch_bounds__(256) void bp_adjoint_kernel(BpArgs A)
+// ------------------------------------------------------------------ transpose
+// BP^T: sino[v][c] = sum over FOV pixels p of img[p] w_p hat_c(xi_p), the transpose of
+// the two-tap interpolation above (clamped taps folded into the edge columns, the
+// 0 < xi < nCols cut kept; w_p = 1 or 1/(U^2+t^2)), gathered by one thread per (view,
+// column), so every sum has a fixed order and no atomics are needed.
+//
+// Column c's taps come from the pixels whose column coordinate lies within one column of
+// its node xi = c + 1/2, i.e. between the node rays of columns c - 1 and c + 1.  The
+// thread marches along the axis its node ray runs closest to (theta = beta + gamma_c; y if
+// |sin theta| >= |cos theta|, else x).  On line k of that axis the node ray crosses the
+// other axis at pixel coordinate m*(k) = a k + b (one FP64 FMA), and the candidates are
+// the pixels within hw(k) = L*(k) tan|dCol| / ((|pd| - |pl| tan|dCol|) h) of m*(k), cut to
+// the line's FOV range.  Each candidate's position relative to the node ray comes from
+// its small offset Delta = (m - m*(k)) h along the line: perpendicular distance
+// d = Delta pd, distance from the source L = L*(k) + Delta pl, and
+// (xi - 1/2) - c = atan(d/L) / dCol = z (1 - z^2/3) / dCol, z = d/L (|z| is at most about
+// tan|dCol|, so the next term is below 1e-9 column).  Every quantity is small, so the
+// column offset is accurate to a few 1e-7 column, where an FP32 column coordinate carries
+// half an ulp of nCols (3e-5 column at 888 columns); that error, amplified by the ramp
+// filter that follows in FBP^T, is what a scatter with the forward kernels' coordinate
+// would leave.
+struct BpaArgs {
+    const float *img;
+    float *sino;
+    const double2 *csd;     // [nViews] cos, sin beta
+    const double2 *col_csd; // [nCols] cos, sin gamma_c
+    const int2 *fov_row, *fov_col;
+    long long img_stride, sino_stride;
+    int nX, nY, nViews, nCols;
+    double R, xmin, ymin, dx, dy;
+    float rFOV, tanc, icol;  // rFOV, tan|dCol|, 1/dCol
+};
+
+constexpr int kBpaViews = 4;  // views per workgroup (one per wave), 64 columns per wave
+
+template <bool kWeighted, int S>
+__global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int slice0)
 {
-    extern __shared__ float row[];
-    const int v = blockIdx.x, sl = blockIdx.z;
-    for (int c = threadIdx.x; c < A.nCols; c += 256) row[c] = 0.0f;
-    __syncthreads();
-    const float *img = A.img + (size_t)sl * A.img_stride;
-    const int npix = A.nX * A.nY;
-    for (int p = threadIdx.x; p < npix; p += 256) {
-        const int i = p % A.nX, j = p / A.nX;
-        const PixelGeo pg = pixel_geo(A, i, j);
-        if (!(pg.x * pg.x + pg.y * pg.y <= A.r2)) continue;
-        const float val0 = img[p];
-        if (val0 == 0.0f) continue;
-        float xi, U, t;
-        if (!column_of<kNc>(A, pg, v, xi, U, t)) continue;
-        float al;
-        const int c0 = interp_pos(xi, A.nCols, al);
-        const float val = kWeighted ? val0 * __builtin_amdgcn_rcpf(fmaf(U, U, t * t)) : val0;
-        const float w1 = val * al;
-        atomicAdd(&row[c0], val - w1);
-        atomicAdd(&row[min(c0 + 1, A.nCols - 1)], w1);
+    unsigned t, g;  // XCD-aware (tile, slice group)
+    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
+    const int c = (int)(t % gridDim.x) * 64 + ((int)threadIdx.x & 63);
+    const int v = (int)(t / gridDim.x) * kBpaViews + ((int)threadIdx.x >> 6);
+    if (c >= A.nCols || v >= A.nViews) return;  // no barriers below
+    const size_t sl = (size_t)slice0 + (size_t)S * g;
+
+    // node ray of column c in view v: d(x, y) = x sin th - y cos th - R sin gam,
+    // L(x, y) = R cos gam - x cos th - y sin th
+    const double2 vb = A.csd[v], cg = A.col_csd[c];
+    const double cth = vb.x * cg.x - vb.y * cg.y, sth = vb.y * cg.x + vb.x * cg.y;
+    const double Rs = A.R * cg.y, Rc = A.R * cg.x;
+    const bool along_y = fabs(sth) >= fabs(cth);
+    double a, b, L0, Ls, p0, ext;
+    float pd, pl, h;
+    int n_major, s_major, s_minor;
+    const int2 *fov;
+    if (along_y) {  // lines are image rows j; candidates along x
+        const double cot = cth / sth, y0 = A.ymin + 0.5 * A.dy, xs0 = cot * y0 + Rs / sth;
+        a = cot * A.dy / A.dx;
+        b = (xs0 - A.xmin) / A.dx - 0.5;
+        L0 = Rc - cth * xs0 - sth * y0;
+        Ls = -A.dy / sth;
+        pd = (float)sth, pl = (float)(-cth), h = (float)A.dx;
+        n_major = A.nY, s_major = A.nX, s_minor = 1, fov = A.fov_row;
+        p0 = -Rs * cth, ext = fabs(sth);
+    } else {  // lines are image columns i; candidates along y
+        const double tn = sth / cth, x0 = A.xmin + 0.5 * A.dx, ys0 = tn * x0 - Rs / cth;
+        a = tn * A.dx / A.dy;
+        b = (ys0 - A.ymin) / A.dy - 0.5;
+        L0 = Rc - cth * x0 - sth * ys0;
+        Ls = -A.dx / cth;
+        pd = (float)(-cth), pl = (float)(-sth), h = (float)A.dy;
+        n_major = A.nX, s_major = 1, s_minor = A.nX, fov = A.fov_col;
+        p0 = Rs * sth, ext = fabs(cth);
     }
-    __syncthreads();
-    float *out = const_cast<float *>(A.sino) + (size_t)sl * A.sino_stride + (size_t)v * A.nCols;
-    for (int c = threadIdx.x; c < A.nCols; c += 256) out[c] = row[c];
+    // lines holding candidates: points within delta of the node ray (delta bounds every
+    // candidate's |d|) and inside the FOV disk have |s| <= H along the ray from its foot
+    // Rs (sin th, -cos th); their coordinate on the marching axis is p0 + s dir +- delta
+    const double delta = ((double)A.R + A.rFOV) * A.tanc * 1.01 + 2.0 * fmax(A.dx, A.dy);
+    const double dn = fmax(fabs(Rs) - delta, 0.0);
+    const double H2 = (double)A.rFOV * A.rFOV - dn * dn;
+    if (!(H2 > 0.0)) {
+        for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = 0.0f;
+        return;
+    }
+    const double span = sqrt(H2) * ext + delta * (along_y ? fabs(cth) : fabs(sth));
+    const double org = along_y ? A.ymin : A.xmin, step = along_y ? A.dy : A.dx;
+    const int klo = max((int)floor((p0 - span - org) / step - 0.5) - 1, 0);
+    const int khi = min((int)ceil((p0 + span - org) / step - 0.5) + 1, n_major - 1);
+
+    const float L0f = (float)L0, Lsf = (float)Ls;
+    const float apd = fabsf(pd) - fabsf(pl) * A.tanc;
+    const float hwc = apd > 0.05f ? A.tanc / (apd * h) * 1.0001f : 1e9f;
+    const float icol = A.icol, c3 = -A.icol * (1.0f / 3.0f);
+    // hat of column c in rel = (xi - 1/2) - c: 1 - |rel| on (-1, 1); the edge columns take
+    // the clamped taps (weight 1 for xi - 1/2 beyond their node) up to the cut 0 < xi < nCols
+    const float clo = c == 0 ? 0.0f : -3.0e38f, chi = c == A.nCols - 1 ? 0.0f : 3.0e38f;
+    const float cut_lo = c == 0 ? -0.5f : -1.0f, cut_hi = c == A.nCols - 1 ? 0.5f : 1.0f;
+
+    const float *img = A.img + sl * A.img_stride;
+    float acc[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+    for (int k = klo; k <= khi; ++k) {
+        const int2 fr = fov[k];
+        const double ms = fma(a, (double)k, b);
+        const double mfl = floor(ms);
+        const int mi = (int)mfl;
+        const float mf = (float)(ms - mfl);
+        const float Lk = fmaf((float)k, Lsf, L0f);
+        const float hw = fmaf(Lk, hwc, 1.0e-3f);
+        const int lo = max(fr.x, mi + (int)ceilf(mf - hw));
+        const int hi = min(fr.y, mi + (int)floorf(mf + hw));
+        const int base = k * s_major;
+        for (int m = lo; m <= hi; ++m) {
+            const float dl = ((float)(m - mi) - mf) * h;
+            const float d = dl * pd;
+            const float L = fmaf(dl, pl, Lk);
+            const float z = d * __builtin_amdgcn_rcpf(L);
+            const float rel = z * fmaf(z * z, c3, icol);
+            float w = 1.0f - fabsf(fminf(fmaxf(rel, clo), chi));
+            w = (rel > cut_lo && rel < cut_hi) ? w : 0.0f;
+            if constexpr (kWeighted) w *= __builtin_amdgcn_rcpf(fmaf(L, L, d * d));
+            const float *px = img + base + m * s_minor;
+            float q[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) q[s] = px[(size_t)s * A.img_stride];
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[s] = fmaf(w, q[s], acc[s]);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = acc[s];
 }
 
 BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img)
@@ -327,7 +432,7 @@ BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, f
     A.ymin = p.g.ymin;
     A.dx = p.g.dx;
     A.dy = p.g.dy;
-    A.r2 = p.d.r2;
+    A.fov_row = t.fov_row;
     A.xi_scale = p.d.bp_xi_scale;
     A.xi_offset = p.d.bp_xi_offset;
     A.edge_exact = p.d.bp_edge_exact;
@@ -385,25 +490,49 @@ fbct_status launch_bp(const fbct_plan &p, const DeviceTables &t, const float *si
 fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const float *img, float *sino, int n,
                               bool weighted, hipStream_t stream)
 {
-    // bp_adjoint_kernel reads the image through A.img and writes the sinogram through A.sino.
-    BpArgs A = make_args(p, t, sino, const_cast<float *>(img));
-    const size_t lds = (size_t)p.g.nCols * sizeof(float);
-    if (lds > 64 * 1024) return fail(FBCT_ERR_UNSUPPORTED, "fbct_bp_adjoint: detector row too wide for LDS");
-    dim3 grid(p.g.nViews, 1, n);
-    const int nc = p.d.bp_atan_nc;
-#define FBCT_BPA(W, NC) hipLaunchKernelGGL((bp_adjoint_kernel<W, NC>), grid, dim3(256), lds, stream, A)
-    if (weighted) {
-        if (nc == 9) FBCT_BPA(true, 9);
-        else if (nc == 5) FBCT_BPA(true, 5);
-        else FBCT_BPA(true, 0);
-    } else {
-        if (nc == 9) FBCT_BPA(false, 9);
-        else if (nc == 5) FBCT_BPA(false, 5);
-        else FBCT_BPA(false, 0);
-    }
+    BpaArgs A;
+    A.img = img;
+    A.sino = sino;
+    A.csd = t.view_csd;
+    A.col_csd = t.col_csd;
+    A.fov_row = t.fov_row;
+    A.fov_col = t.fov_col;
+    A.img_stride = (long long)p.g.nX * p.g.nY;
+    A.sino_stride = (long long)p.g.nViews * p.g.nCols;
+    A.nX = p.g.nX;
+    A.nY = p.g.nY;
+    A.nViews = p.g.nViews;
+    A.nCols = p.g.nCols;
+    A.R = p.g.R;
+    A.xmin = p.g.xmin;
+    A.ymin = p.g.ymin;
+    A.dx = p.g.dx;
+    A.dy = p.g.dy;
+    A.rFOV = p.g.rFOV;
+    A.tanc = p.d.bpa_tan;
+    A.icol = p.d.bp_xi_scale;
+    // slice groups of 16/8/4/2 while they leave enough workgroups to fill the chip
+    const dim3 tiles((p.g.nCols + 63) / 64, (p.g.nViews + kBpaViews - 1) / kBpaViews);
+    const long long nt = (long long)tiles.x * tiles.y;
+    int done = 0;
+    for (int S : {16, 8, 4, 2, 1}) {
+        const int groups = (n - done) / S;
+        if (groups == 0 || (S > 1 && nt * groups < 2048)) continue;
+        const dim3 grid(tiles.x, tiles.y, groups);
+#define FBCT_BPA(W, SS) hipLaunchKernelGGL((bp_adjoint_gather_kernel<W, SS>), grid, dim3(256), 0, stream, A, done)
+        switch (S) {
+        case 16: if (weighted) FBCT_BPA(true, 16); else FBCT_BPA(false, 16); break;
+        case 8: if (weighted) FBCT_BPA(true, 8); else FBCT_BPA(false, 8); break;
+        case 4: if (weighted) FBCT_BPA(true, 4); else FBCT_BPA(false, 4); break;
+        case 2: if (weighted) FBCT_BPA(true, 2); else FBCT_BPA(false, 2); break;
+        default: if (weighted) FBCT_BPA(true, 1); else FBCT_BPA(false, 1); break;
+        }
 #undef FBCT_BPA
+        done += groups * S;
+        if (done == n) break;
+    }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_adjoint_kernel launch: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_adjoint_gather_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

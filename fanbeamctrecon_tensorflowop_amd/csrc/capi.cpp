@@ -35,9 +35,31 @@ fbct_status filter_rows(const fbct_plan &p, const fbct::DeviceTables &t, const f
                                     : fbct::launch_ramp_filter_mfma(p, t, in, out, rows, st);
 }
 
-size_t sino_bytes(const fbct_plan *p, int64_t n)
+size_t sino_bytes(const fbct_plan *p, int64_t n, int dtype = FBCT_F32)
 {
-    return static_cast<size_t>(n) * p->g.nViews * p->g.nCols * sizeof(float);
+    return static_cast<size_t>(n) * p->g.nViews * p->g.nCols * (dtype == FBCT_BF16 ? 2 : 4);
+}
+
+size_t image_bytes(const fbct_plan *p, int64_t n, int dtype = FBCT_F32)
+{
+    return static_cast<size_t>(n) * p->g.nX * p->g.nY * (dtype == FBCT_BF16 ? 2 : 4);
+}
+
+// The image and sinogram buffers of one call must not overlap.
+fbct_status check_io(const fbct_plan *p, const void *image, int image_dtype, const void *sino, int sino_dtype,
+                     int64_t n, const char *what)
+{
+    return fbct::check_disjoint(image, image_bytes(p, n, image_dtype), sino, sino_bytes(p, n, sino_dtype), what,
+                                "image", "sinogram");
+}
+
+// The FBP workspace (the filtered sinograms) must overlap neither the sinogram nor the image.
+fbct_status check_workspace(const fbct_plan *p, const void *image, const void *sino, const void *ws, int64_t n,
+                            const char *what)
+{
+    fbct_status s = fbct::check_disjoint(ws, sino_bytes(p, n), sino, sino_bytes(p, n), what, "workspace", "sinogram");
+    if (s != FBCT_OK) return s;
+    return fbct::check_disjoint(ws, sino_bytes(p, n), image, image_bytes(p, n), what, "workspace", "image");
 }
 
 }  // namespace
@@ -48,6 +70,7 @@ fbct_status fbct_fp(fbct_plan *plan, const float *image, float *sino, int n_slic
 {
     fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fp");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fp")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_fp(*plan, *t, image, FBCT_F32, sino, FBCT_F32, n_slices, static_cast<hipStream_t>(stream));
@@ -62,6 +85,10 @@ fbct_status fbct_fp_residual(fbct_plan *plan, const void *image, int image_dtype
     if ((s = check_dtype(image_dtype, "fbct_fp_residual")) != FBCT_OK ||
         (s = check_dtype(residual_dtype, "fbct_fp_residual")) != FBCT_OK)
         return s;
+    if ((s = check_io(plan, image, image_dtype, residual, residual_dtype, n_slices, "fbct_fp_residual")) != FBCT_OK ||
+        (s = fbct::check_disjoint(target, sino_bytes(plan, n_slices), residual, sino_bytes(plan, n_slices, residual_dtype),
+                                  "fbct_fp_residual", "target", "residual")) != FBCT_OK)
+        return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_fp(*plan, *t, image, static_cast<fbct_dtype>(image_dtype), residual,
@@ -75,6 +102,7 @@ fbct_status fbct_fp_ex(fbct_plan *plan, const void *image, int image_dtype, void
     if (s != FBCT_OK || n_slices == 0) return s;
     if ((s = check_dtype(image_dtype, "fbct_fp_ex")) != FBCT_OK || (s = check_dtype(sino_dtype, "fbct_fp_ex")) != FBCT_OK)
         return s;
+    if ((s = check_io(plan, image, image_dtype, sino, sino_dtype, n_slices, "fbct_fp_ex")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_fp(*plan, *t, image, static_cast<fbct_dtype>(image_dtype), sino,
@@ -85,6 +113,7 @@ fbct_status fbct_fp_adjoint(fbct_plan *plan, const float *sino, float *image, in
 {
     fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fp_adjoint");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fp_adjoint")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_fp_adjoint(*plan, *t, sino, FBCT_F32, image, FBCT_F32, n_slices,
@@ -99,6 +128,7 @@ fbct_status fbct_fp_adjoint_ex(fbct_plan *plan, const void *sino, int sino_dtype
     if ((s = check_dtype(sino_dtype, "fbct_fp_adjoint_ex")) != FBCT_OK ||
         (s = check_dtype(image_dtype, "fbct_fp_adjoint_ex")) != FBCT_OK)
         return s;
+    if ((s = check_io(plan, image, image_dtype, sino, sino_dtype, n_slices, "fbct_fp_adjoint_ex")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_fp_adjoint(*plan, *t, sino, static_cast<fbct_dtype>(sino_dtype), image,
@@ -109,6 +139,7 @@ fbct_status fbct_bp(fbct_plan *plan, const float *sino, float *image, int n_slic
 {
     fbct_status s = check_common(plan, sino, image, n_slices, "fbct_bp");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_bp")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_bp(*plan, *t, sino, image, n_slices, false, static_cast<hipStream_t>(stream));
@@ -118,6 +149,7 @@ fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, in
 {
     fbct_status s = check_common(plan, image, sino, n_slices, "fbct_bp_adjoint");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_bp_adjoint")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, false, static_cast<hipStream_t>(stream));
@@ -127,6 +159,7 @@ fbct_status fbct_fbp_backproject(fbct_plan *plan, const float *sino, float *imag
 {
     fbct_status s = check_common(plan, sino, image, n_slices, "fbct_fbp_backproject");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fbp_backproject")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_bp(*plan, *t, sino, image, n_slices, true, static_cast<hipStream_t>(stream));
@@ -137,6 +170,7 @@ fbct_status fbct_fbp_backproject_adjoint(fbct_plan *plan, const float *image, fl
 {
     fbct_status s = check_common(plan, image, sino, n_slices, "fbct_fbp_backproject_adjoint");
     if (s != FBCT_OK || n_slices == 0) return s;
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fbp_backproject_adjoint")) != FBCT_OK) return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_bp_adjoint(*plan, *t, image, sino, n_slices, true, static_cast<hipStream_t>(stream));
@@ -183,6 +217,9 @@ fbct_status fbct_fbp_ex(fbct_plan *plan, const float *sino, float *image, int n_
     if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
         return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_ex: workspace of %zu bytes needed, got %zu",
                     sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fbp_ex")) != FBCT_OK ||
+        (s = check_workspace(plan, image, sino, workspace, n_slices, "fbct_fbp_ex")) != FBCT_OK)
+        return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -222,6 +259,9 @@ fbct_status fbct_fbp_adjoint(fbct_plan *plan, const float *image, float *sino, i
     if (!workspace || workspace_bytes < sino_bytes(plan, n_slices))
         return fail(FBCT_ERR_WORKSPACE, "fbct_fbp_adjoint: workspace of %zu bytes needed, got %zu",
                     sino_bytes(plan, n_slices), workspace ? workspace_bytes : (size_t)0);
+    if ((s = check_io(plan, image, FBCT_F32, sino, FBCT_F32, n_slices, "fbct_fbp_adjoint")) != FBCT_OK ||
+        (s = check_workspace(plan, image, sino, workspace, n_slices, "fbct_fbp_adjoint")) != FBCT_OK)
+        return s;
     const fbct::DeviceTables *t = nullptr;
     if ((s = fbct::device_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -219,6 +219,16 @@ fbct_status check_dtype(int d, const char *what)
     return FBCT_OK;
 }
 
+size_t proj_bytes(const fbct_cone_plan *p, int n, int dtype)
+{
+    return (size_t)n * p->g.nViews * p->g.nRows * p->g.nCols * (dtype == FBCT_BF16 ? 2 : 4);
+}
+
+size_t vol_bytes(const fbct_cone_plan *p, int n, int dtype)
+{
+    return (size_t)n * p->g.nZ * p->g.nY * p->g.nX * (dtype == FBCT_BF16 ? 2 : 4);
+}
+
 fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void *proj, fbct_dtype ti, float *q, int n,
                        hipStream_t st)
 {
@@ -325,6 +335,9 @@ fbct_status fbct_cone_filter(fbct_cone_plan *plan, const void *proj, int proj_dt
     fbct_status s = fbct::check_exec(plan, proj, filtered, n, "fbct_cone_filter");
     if (s != FBCT_OK || n == 0) return s;
     if ((s = fbct::check_dtype(proj_dtype, "fbct_cone_filter")) != FBCT_OK) return s;
+    if ((s = fbct::check_disjoint(proj, fbct::proj_bytes(plan, n, proj_dtype), filtered,
+                                  fbct::proj_bytes(plan, n, FBCT_F32), "fbct_cone_filter", "proj", "filtered")) != FBCT_OK)
+        return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::run_filter(plan, *t, proj, static_cast<fbct_dtype>(proj_dtype), filtered, n,
@@ -337,6 +350,9 @@ fbct_status fbct_cone_backproject(fbct_cone_plan *plan, const float *filtered, v
     fbct_status s = fbct::check_exec(plan, filtered, vol, n, "fbct_cone_backproject");
     if (s != FBCT_OK || n == 0) return s;
     if ((s = fbct::check_dtype(vol_dtype, "fbct_cone_backproject")) != FBCT_OK) return s;
+    if ((s = fbct::check_disjoint(filtered, fbct::proj_bytes(plan, n, FBCT_F32), vol, fbct::vol_bytes(plan, n, vol_dtype),
+                                  "fbct_cone_backproject", "filtered", "vol")) != FBCT_OK)
+        return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
     return fbct::launch_cone_bp(*plan, *t, filtered, vol, static_cast<fbct_dtype>(vol_dtype), n,
@@ -355,6 +371,13 @@ fbct_status fbct_cone_fdk(fbct_cone_plan *plan, const void *proj, int proj_dtype
     if (!workspace || workspace_bytes < need)
         return fbct::fail(FBCT_ERR_WORKSPACE, "fbct_cone_fdk: workspace of %zu bytes needed, got %zu", need,
                           workspace ? workspace_bytes : (size_t)0);
+    if ((s = fbct::check_disjoint(proj, fbct::proj_bytes(plan, n, proj_dtype), vol, fbct::vol_bytes(plan, n, vol_dtype),
+                                  "fbct_cone_fdk", "proj", "vol")) != FBCT_OK ||
+        (s = fbct::check_disjoint(workspace, need, proj, fbct::proj_bytes(plan, n, proj_dtype), "fbct_cone_fdk",
+                                  "workspace", "proj")) != FBCT_OK ||
+        (s = fbct::check_disjoint(workspace, need, vol, fbct::vol_bytes(plan, n, vol_dtype), "fbct_cone_fdk",
+                                  "workspace", "vol")) != FBCT_OK)
+        return s;
     const fbct::ConeDeviceTables *t = nullptr;
     if ((s = fbct::cone_tables(plan, &t)) != FBCT_OK) return s;
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -109,6 +109,9 @@ struct DeviceTables {
     int32_t *fp_order[kFpNumCfg] = {};  // per bundle config: items v*nb+b by decreasing tile rows
     float2 *view_cs = nullptr;     // [nViews] (cos beta, sin beta)
     double2 *view_csd = nullptr;   // [nViews] (cos beta, sin beta) in FP64
+    double2 *col_csd = nullptr;    // [nCols] (cos gamma_c, sin gamma_c) in FP64, gamma_c of the BP column map
+    int2 *fov_row = nullptr;       // [nY] pixels i in [x, y] of row j lie in the FOV disk (x > y: none)
+    int2 *fov_col = nullptr;       // [nX] pixels j in [x, y] of column i lie in the FOV disk
     double *ramp = nullptr;        // [2*nCols-1]
     uint16_t *ramp_planes = nullptr;  // [3][Np][Kp] BF16 split planes of the Toeplitz matrix H
     float *parker = nullptr;       // [nViews][nCols] short-scan weights (uploaded on first use)
@@ -123,6 +126,7 @@ struct Derived {
     float bp_xi_scale;    // 1/dCol
     float bp_xi_offset;   // dShift - dLeft/dCol
     float r2;             // rFOV*rFOV (float, as the reference)
+    float bpa_tan;        // tan |dCol|: BP^T candidate half-width per unit distance from the source
     // In-range decision 0 < xi < nCols taken in FP64 near the two edges: with
     // g = atan(t/U), the pixel is inside iff tan_min < t/U < tan_max.
     int bp_edge_exact;    // 1 when the fan edges lie within (-pi/2, pi/2)
@@ -156,6 +160,12 @@ struct fbct_plan {
     std::vector<int16_t> fp_order_rows[fbct::kFpNumCfg];
     std::vector<float> view_cs;     // 2 per view
     std::vector<double> view_csd;   // 2 per view
+    std::vector<double> col_csd;    // 2 per column: cos, sin of gamma_c = dLeft + (c + 1/2 - dShift) dCol
+    // FOV disk membership of the pixel centres, x^2 + y^2 <= rFOV^2 in FP64 (the reference
+    // tests it in FP32, BP/transform.cu:80-81; the two can differ only for centres within
+    // rounding of the circle): per row the range of columns [lo, hi], per column the range
+    // of rows (lo > hi: empty)
+    std::vector<int32_t> fov_row, fov_col;
     std::vector<double> ramp;       // 2*nCols-1
     std::vector<uint16_t> ramp_planes;  // BF16 hi/lo/ll planes of H, [3][Np][Kp]
     int64_t fp_samples = 0;
@@ -242,6 +252,17 @@ namespace fbct {
 // Thread-local error reporting.
 fbct_status fail(fbct_status s, const char *fmt, ...);
 fbct_status ok();
+
+// FBCT_ERR_INVALID_ARGUMENT when the byte ranges [a, a + na) and [b, b + nb) share a byte
+// (an execute call whose kernels read one buffer while others write the second would
+// return wrong data).
+inline fbct_status check_disjoint(const void *a, size_t na, const void *b, size_t nb, const char *what,
+                                  const char *aname, const char *bname)
+{
+    const char *x = static_cast<const char *>(a), *y = static_cast<const char *>(b);
+    if (na && nb && x < y + nb && y < x + na) return fail(FBCT_ERR_INVALID_ARGUMENT, "%s: %s and %s must not overlap", what, aname, bname);
+    return FBCT_OK;
+}
 
 // Host helpers shared by the plan kinds (plan.cpp).
 uint16_t bf16_rne(float f);

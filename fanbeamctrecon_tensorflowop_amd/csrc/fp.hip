@@ -107,7 +107,7 @@ extern __shared__ float fp_lds[];
 // Bundle b of view v for slices [s0, s0+S): every thread of the workgroup calls it with
 // the same arguments (it contains barriers).
 template <int S, int G, typename TI, typename TO>
-__device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0, int *err)
+__device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
 {
     using TapT = typename Taps<S>::T;
     constexpr int kSlab = kFpCfgSlab[fp_cfg(G)];
@@ -189,10 +189,10 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
         kcarry = __builtin_amdgcn_ds_swizzle(kown, 0x1f | ((G - 1) << 10));
         if (mlo > mhi) continue;  // no samples of this bundle in the slab
         const int M = mhi - mlo + 1;
-        if (M > A.cap) {  // cannot happen for tables built by the plan; never read past LDS
-            if (tid == 0) atomicOr(err, 1);
-            continue;
-        }
+        // M <= A.cap for every item of a launch: the plan (and the batch) check their row
+        // tables against the caps the launches use when they are built; kept so that LDS
+        // is never addressed past the tile whatever the tables hold
+        if (M > A.cap) continue;
         __syncthreads();  // the previous slab's reads of `tile` are done
 
         // Stage taps [alo, alo+kSlab] x [mlo, mhi] of the S slices with clamp-to-edge
@@ -286,13 +286,12 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0,
 
 // Items order[item0 + t] (t < gridDim.x) of slice group blockIdx.y; item = v * nb + b.
 template <int S, int G, typename TI, typename TO>
-__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, const int32_t *__restrict__ order, int item0,
-                                                        int *err)
+__global__ __launch_bounds__(kFpThreads) void fp_kernel(FpArgs A, const int32_t *__restrict__ order, int item0)
 {
     unsigned t, g;  // XCD-aware (item, slice group)
     xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x, gridDim.y, t, g);
     const int vb = order[item0 + (int)t];
-    fp_bundle<S, G, TI, TO>(A, vb % A.nb, vb / A.nb, g * S, err);
+    fp_bundle<S, G, TI, TO>(A, vb % A.nb, vb / A.nb, g * S);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, view,
@@ -303,7 +302,7 @@ template <int G, typename TI, typename TO>
 __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc *__restrict__ desc,
                                                               const int *__restrict__ prefix, int n,
                                                               const int32_t *__restrict__ order, int count, int cap,
-                                                              const TI *img, TO *sino, int *err)
+                                                              const TI *img, TO *sino)
 {
     constexpr int cfg = fp_cfg(G);
     for (int t = blockIdx.x; t < count; t += gridDim.x) {
@@ -336,52 +335,27 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.cap = min(d.cap[cfg], cap);  // the launch's tile; the items it is given fit it
         A.scale = d.scale;
         const int local = item - prefix[lo];
-        fp_bundle<1, G, TI, TO>(A, local % A.nb, local / A.nb, 0, err);
+        fp_bundle<1, G, TI, TO>(A, local % A.nb, local / A.nb, 0);
         __syncthreads();  // the next item reuses the LDS tile
     }
 }
 
-__device__ int g_fp_err = 0;
-
 }  // namespace
 
-// Device address of g_fp_err on the current device, looked up once per device (so no
-// runtime symbol lookup happens while a caller captures a graph).
-static fbct_status fp_err_ptr(int **out)
-{
-    static std::mutex mu;
-    static int *ptrs[64] = {};
-    int dev = -1;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
-    if (dev < 0 || dev >= 64) return fail(FBCT_ERR_UNSUPPORTED, "device index %d", dev);
-    std::lock_guard<std::mutex> lk(mu);
-    if (!ptrs[dev]) {
-        e = hipGetSymbolAddress(reinterpret_cast<void **>(&ptrs[dev]), HIP_SYMBOL(g_fp_err));
-        if (e != hipSuccess) {
-            ptrs[dev] = nullptr;
-            return fail(FBCT_ERR_DEVICE, "hipGetSymbolAddress: %s", hipGetErrorString(e));
-        }
-    }
-    *out = ptrs[dev];
-    return FBCT_OK;
-}
-
 template <int S, int G, typename TI, typename TO>
-static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, const int32_t *order, int item0,
-                        int *err)
+static void launch_fp_s(dim3 grid, size_t lds, hipStream_t stream, const FpArgs &A, const int32_t *order, int item0)
 {
-    hipLaunchKernelGGL((fp_kernel<S, G, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, order, item0, err);
+    hipLaunchKernelGGL((fp_kernel<S, G, TI, TO>), grid, dim3(kFpThreads), lds, stream, A, order, item0);
 }
 
 template <int S, int G>
 static fbct_status launch_fp_t(fbct_dtype ti, fbct_dtype to, dim3 grid, size_t lds, hipStream_t stream,
-                               const FpArgs &A, const int32_t *order, int item0, int *err)
+                               const FpArgs &A, const int32_t *order, int item0)
 {
-    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, G, float, float>(grid, lds, stream, A, order, item0, err);
-    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, G, bf16_t, float>(grid, lds, stream, A, order, item0, err);
-    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, G, float, bf16_t>(grid, lds, stream, A, order, item0, err);
-    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, G, bf16_t, bf16_t>(grid, lds, stream, A, order, item0, err);
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_s<S, G, float, float>(grid, lds, stream, A, order, item0);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_s<S, G, bf16_t, float>(grid, lds, stream, A, order, item0);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_s<S, G, float, bf16_t>(grid, lds, stream, A, order, item0);
+    else if (ti == FBCT_BF16 && to == FBCT_BF16) launch_fp_s<S, G, bf16_t, bf16_t>(grid, lds, stream, A, order, item0);
     else return fail(FBCT_ERR_INVALID_ARGUMENT, "fbct_fp: unknown storage type");
     return FBCT_OK;
 }
@@ -403,8 +377,6 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
     A.nCols = p.g.nCols;
     A.scale = p.d.fp_scale;
     A.nSlices = n;
-    int *err = nullptr;
-    if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
     hipError_t e;
 
     // Slices per workgroup S and bundle configuration: share each sample's position and
@@ -453,12 +425,12 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
         const dim3 grid((unsigned)count, groups);
         fbct_status st;
         if (cfg == 0) {
-            if (S == 2) st = launch_fp_t<2, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
-            else st = launch_fp_t<1, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+            if (S == 2) st = launch_fp_t<2, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0);
+            else st = launch_fp_t<1, 4>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0);
         } else {
-            if (S == 4) st = launch_fp_t<4, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
-            else if (S == 2) st = launch_fp_t<2, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
-            else st = launch_fp_t<1, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0, err);
+            if (S == 4) st = launch_fp_t<4, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0);
+            else if (S == 2) st = launch_fp_t<2, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0);
+            else st = launch_fp_t<1, 8>(ti, to, grid, lds, stream, A, t.fp_order[cfg], item0);
         }
         if (st != FBCT_OK) return st;
         e = hipGetLastError();
@@ -469,13 +441,13 @@ fbct_status launch_fp(const fbct_plan &p, const DeviceTables &t, const void *img
 
 template <int G, typename TI, typename TO>
 static void launch_fp_batch_t(dim3 grid, size_t lds, hipStream_t stream, const fbct_batch &b, const BatchDevice &bd,
-                              const int32_t *order, int count, int cap, const void *img, void *sino, int *err)
+                              const int32_t *order, int count, int cap, const void *img, void *sino)
 {
     constexpr int cfg = fp_cfg(G);
     const int n = (int)b.plans.size();
     hipLaunchKernelGGL((fp_batch_kernel<G, TI, TO>), grid, dim3(kFpThreads), lds, stream, bd.fp_desc,
                        bd.fp_prefix[cfg], n, order, count, cap, static_cast<const TI *>(img),
-                       static_cast<TO *>(sino), err);
+                       static_cast<TO *>(sino));
 }
 
 fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const void *img, fbct_dtype ti, void *sino,
@@ -485,8 +457,6 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
     const int cfg = b.shape.fp_rays == 32 ? 1 : 0;
     const int total = b.fp_prefix[cfg][n];
     if (total == 0) return FBCT_OK;
-    int *err = nullptr;
-    if (fbct_status es = fp_err_ptr(&err); es != FBCT_OK) return es;
     hipError_t e;
     const size_t row_bytes = (size_t)fp_pitch(cfg) * sizeof(float);
     if ((size_t)b.fp_cap_max[cfg] * row_bytes > 64 * 1024)
@@ -509,10 +479,10 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
         const int32_t *order = bd.fp_order[cfg] + item0;
 #define FBCT_FPB(G)                                                                                       \
     do {                                                                                                  \
-        if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_batch_t<G, float, float>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
-        else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_batch_t<G, bf16_t, float>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
-        else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_batch_t<G, float, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino, err); \
-        else launch_fp_batch_t<G, bf16_t, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino, err);          \
+        if (ti == FBCT_F32 && to == FBCT_F32) launch_fp_batch_t<G, float, float>(grid, lds, stream, b, bd, order, count, cap, img, sino); \
+        else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fp_batch_t<G, bf16_t, float>(grid, lds, stream, b, bd, order, count, cap, img, sino); \
+        else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fp_batch_t<G, float, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino); \
+        else launch_fp_batch_t<G, bf16_t, bf16_t>(grid, lds, stream, b, bd, order, count, cap, img, sino);          \
     } while (0)
         if (cfg == 0) FBCT_FPB(4);
         else FBCT_FPB(8);

@@ -424,6 +424,51 @@ static void build_tables(fbct_plan &p)
     d.bp_xi_offset = static_cast<float>(static_cast<double>(g.dShift) -
                                         static_cast<double>(g.dLeft) / static_cast<double>(g.dCol));
     d.r2 = g.rFOV * g.rFOV;
+
+    // BP^T gathers per (view, column): gamma_c of every column in FP64 and the FOV disk
+    // as pixel ranges per row and per column (FP64 membership test of the centres).
+    p.col_csd.resize(static_cast<size_t>(2) * nC);
+    for (int c = 0; c < nC; ++c) {
+        const double gam = static_cast<double>(g.dLeft) +
+                           (static_cast<double>(c) + 0.5 - static_cast<double>(g.dShift)) * static_cast<double>(g.dCol);
+        p.col_csd[2 * c] = std::cos(gam);
+        p.col_csd[2 * c + 1] = std::sin(gam);
+    }
+    {
+        // a line meets the disk in one interval: start from the FP64 chord and settle each
+        // end with the membership test itself
+        const double rf2 = static_cast<double>(g.rFOV) * static_cast<double>(g.rFOV);
+        auto ranges = [rf2](int n_lines, double lmin, double dl, int n, double pmin, double dp,
+                            std::vector<int32_t> &out) {
+            out.assign(static_cast<size_t>(2) * n_lines, 0);
+            for (int k = 0; k < n_lines; ++k) {
+                const double y = lmin + dl * (k + 0.5);
+                auto in = [&](int i) {
+                    const double x = pmin + dp * (i + 0.5);
+                    return x * x + y * y <= rf2;
+                };
+                const double h = rf2 - y * y;
+                int lo = n, hi = -1;
+                if (h >= 0.0) {
+                    const double e = std::sqrt(h);
+                    int a = static_cast<int>(std::floor((-e - pmin) / dp - 0.5));
+                    int b = static_cast<int>(std::ceil((e - pmin) / dp - 0.5));
+                    a = std::min(std::max(a, 0), n - 1);
+                    b = std::min(std::max(b, 0), n - 1);
+                    while (a > 0 && in(a - 1)) --a;
+                    while (a <= b && !in(a)) ++a;
+                    while (b < n - 1 && in(b + 1)) ++b;
+                    while (b >= a && !in(b)) --b;
+                    if (a <= b) lo = a, hi = b;
+                }
+                out[2 * k] = lo;
+                out[2 * k + 1] = hi;
+            }
+        };
+        ranges(g.nY, g.ymin, g.dy, g.nX, g.xmin, g.dx, p.fov_row);
+        ranges(g.nX, g.xmin, g.dx, g.nY, g.ymin, g.dy, p.fov_col);
+    }
+    d.bpa_tan = static_cast<float>(std::tan(std::fabs(static_cast<double>(g.dCol))));
     // FOV pixels see the source at |t/U| <= rFOV/sqrt(R^2 - rFOV^2); the margin covers
     // the FP32 pixel centres on the circle.
     {
@@ -514,6 +559,9 @@ static void free_tables(DeviceTables &t)
     (void)hipFree(t.ray_inv);
     (void)hipFree(t.view_cs);
     (void)hipFree(t.view_csd);
+    (void)hipFree(t.col_csd);
+    (void)hipFree(t.fov_row);
+    (void)hipFree(t.fov_col);
     (void)hipFree(t.ramp);
     (void)hipFree(t.ramp_planes);
     (void)hipFree(t.parker);
@@ -574,6 +622,10 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
             *out = &t;
             return FBCT_OK;
         }
+    // Plans are tiny; keep the vector from reallocating so returned pointers stay valid.
+    if (plan->dev.capacity() == 0) plan->dev.reserve(16);
+    if (plan->dev.size() == plan->dev.capacity())
+        return fail(FBCT_ERR_UNSUPPORTED, "plan used on more than %zu devices", plan->dev.capacity());
     DeviceTables t;
     t.device = dev;
     fbct_status s;
@@ -590,6 +642,9 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
         (s = upload(&t.ray_inv, plan->ray_inv.data(), plan->ray_inv.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_cs, plan->view_cs.data(), plan->view_cs.size() * sizeof(float))) != FBCT_OK ||
         (s = upload(&t.view_csd, plan->view_csd.data(), plan->view_csd.size() * sizeof(double))) != FBCT_OK ||
+        (s = upload(&t.col_csd, plan->col_csd.data(), plan->col_csd.size() * sizeof(double))) != FBCT_OK ||
+        (s = upload(&t.fov_row, plan->fov_row.data(), plan->fov_row.size() * sizeof(int32_t))) != FBCT_OK ||
+        (s = upload(&t.fov_col, plan->fov_col.data(), plan->fov_col.size() * sizeof(int32_t))) != FBCT_OK ||
         (s = upload(&t.ramp, plan->ramp.data(), plan->ramp.size() * sizeof(double))) != FBCT_OK ||
         (s = upload(&t.ramp_planes, plan->ramp_planes.data(), plan->ramp_planes.size() * sizeof(uint16_t))) != FBCT_OK) {
         std::string msg = g_last_error;
@@ -597,10 +652,6 @@ fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
         g_last_error = msg;
         return s;
     }
-    // Plans are tiny; keep the vector from reallocating so returned pointers stay valid.
-    if (plan->dev.capacity() == 0) plan->dev.reserve(16);
-    if (plan->dev.size() == plan->dev.capacity())
-        return fail(FBCT_ERR_UNSUPPORTED, "plan used on more than %zu devices", plan->dev.capacity());
     plan->dev.push_back(t);
     *out = &plan->dev.back();
     return FBCT_OK;

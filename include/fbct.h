@@ -22,6 +22,10 @@
  * sequence can be captured into a HIP graph once the plan's tables are resident on
  * the device (fbct_plan_upload, or any earlier execute on that device).
  *
+ * The buffers an execute call reads and the buffers it writes (workspaces included)
+ * must not overlap: kernels read one while other workgroups write the other, so an
+ * overlap is refused with FBCT_ERR_INVALID_ARGUMENT before any device work.
+ *
  * Errors are returned as fbct_status; nothing ever exits the process (the reference
  * exits through checkCudaErrors, FP/transform.cu:4).  fbct_last_error() gives a
  * thread-local message for the last failing call on the calling thread.
@@ -176,7 +180,9 @@ fbct_status fbct_fp_adjoint_ex(fbct_plan *plan, const void *sino, int sino_dtype
  * Replaces backproject_fan_curve of BP (BP/transform.cu:123-176; kernel :67-95). */
 fbct_status fbct_bp(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream);
 
-/* Transpose of fbct_bp.  image -> sino.  No reference counterpart. */
+/* Transpose of fbct_bp.  image -> sino.  No reference counterpart.  One thread per
+ * (view, column) gathers the pixels whose column coordinate lies within one column of
+ * the column's node, so every sum has a fixed order (bitwise repeatable, no atomics). */
 fbct_status fbct_bp_adjoint(fbct_plan *plan, const float *image, float *sino, int n_slices,
                             void *stream);
 

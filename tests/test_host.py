@@ -100,6 +100,48 @@ def test_execute_validation_without_gpu():
     assert lib.fbct_fp_residual(h, fake, 0, fake, fake, 5, 2, null) == 1
 
 
+def test_overlapping_buffers_rejected_without_gpu():
+    """Buffers one call reads and writes must be disjoint: the kernels read one while other
+    workgroups write the other, so aliasing (a workspace passed as the sinogram, an
+    in-place filter) is refused with FBCT_ERR_INVALID_ARGUMENT before any device work."""
+    from fanbeamctrecon_tensorflowop_amd.cone import ConePlan
+    lib = _lib.load()
+    plan = FanBeamPlan(*workloads.tiny_scan())
+    h = plan._h
+    null = ctypes.c_void_p(0)
+    sino_b = plan.n_views * plan.n_cols * 4
+    img_b = plan.nx * plan.ny * 4
+    a, b, c = 0x10000000, 0x20000000, 0x30000000  # three disjoint fake device ranges
+    P = ctypes.c_void_p
+    ws = lib.fbct_fbp_workspace_size(h, 1)
+    # image and sinogram of one call
+    assert lib.fbct_fp(h, P(a), P(a + img_b - 4), 1, null) == 1
+    assert lib.fbct_fp_adjoint(h, P(a), P(a), 1, null) == 1
+    assert lib.fbct_bp(h, P(a), P(a + 8), 1, null) == 1
+    assert lib.fbct_bp_adjoint(h, P(a), P(a), 1, null) == 1
+    assert lib.fbct_fp_ex(h, P(a), 1, P(a + img_b // 2 - 2), 1, 1, null) == 1
+    # the FBP workspace against the sinogram and the image (forward and transpose)
+    assert lib.fbct_fbp(h, P(a), P(b), 1, P(a), ws, null) == 1
+    assert lib.fbct_fbp(h, P(a), P(b), 1, P(b + img_b - 4), ws, null) == 1
+    assert lib.fbct_fbp_adjoint(h, P(b), P(a), 1, P(a + sino_b - 4), ws, null) == 1
+    assert lib.fbct_fbp_adjoint(h, P(b), P(a), 1, P(b), ws, null) == 1
+    assert lib.fbct_fbp_ex(h, P(a), P(b), 1, P(a), ws, 1, null) == 1
+    assert "must not overlap" in _lib.last_error()
+    # the residual against its target
+    assert lib.fbct_fp_residual(h, P(a), 0, P(b), P(b), 0, 1, null) == 1
+    # cone: in-place row filter, workspace aliasing the projections or the volume
+    g, ang = workloads.cone_scan(nx=16, nz=4, dx=1.0, n_views=12, n_cols=24, n_rows=6)
+    cp = ConePlan(g, ang)
+    proj_b = 12 * 6 * 24 * 4
+    vol_b = 4 * 16 * 16 * 4
+    cws = lib.fbct_cone_workspace_size(cp._h, 1)
+    assert lib.fbct_cone_filter(cp._h, P(a), 0, P(a + proj_b // 2), 1, null) == 1
+    assert lib.fbct_cone_backproject(cp._h, P(a), P(a + proj_b - 4), 0, 1, null) == 1
+    assert lib.fbct_cone_fdk(cp._h, P(a), 0, P(b), 0, 1, P(a), cws, null) == 1
+    assert lib.fbct_cone_fdk(cp._h, P(a), 0, P(b), 0, 1, P(b + vol_b - 4), cws, null) == 1
+    assert lib.fbct_cone_fdk(cp._h, P(a), 0, P(a), 0, 1, P(c), cws, null) == 1
+
+
 def test_uneven_batch_host_side():
     """fbct_batch_* creation, layout, work counts and shape validation (no device calls)."""
     from fanbeamctrecon_tensorflowop_amd.batch import UnevenBatch
