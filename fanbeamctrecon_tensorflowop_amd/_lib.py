@@ -11,7 +11,9 @@ import os
 import threading
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG_DIR, "libfbct.so")
+# FBCT_LIBRARY names another build of the same library (the host-sanitizer build of
+# tests/run_sanitizers.py); by default the in-tree libfbct.so is loaded.
+LIB_PATH = os.environ.get("FBCT_LIBRARY") or os.path.join(_PKG_DIR, "libfbct.so")
 
 _lock = threading.Lock()
 _lib = None

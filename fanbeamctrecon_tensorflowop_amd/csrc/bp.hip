@@ -379,9 +379,13 @@ __global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int s
     const float cut_lo = c == 0 ? -0.5f : -1.0f, cut_hi = c == A.nCols - 1 ? 0.5f : 1.0f;
 
     const float *img = A.img + sl * A.img_stride;
-    float acc[S];
+    // FP32 sums over a line's candidates, FP64 across lines: FBP^T applies the ramp
+    // filter to this output, which amplifies its relative rounding about 70-fold on
+    // smooth images, and a plain FP32 sum over a column's ~1500 candidates would carry
+    // ~4e-7 of it
+    double accd[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) acc[s] = 0.0f;
+    for (int s = 0; s < S; ++s) accd[s] = 0.0;
     for (int k = klo; k <= khi; ++k) {
         const int2 fr = fov[k];
         const double ms = fma(a, (double)k, b);
@@ -393,6 +397,9 @@ __global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int s
         const int lo = max(fr.x, mi + (int)ceilf(mf - hw));
         const int hi = min(fr.y, mi + (int)floorf(mf + hw));
         const int base = k * s_major;
+        float acc[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) acc[s] = 0.0f;
         for (int m = lo; m <= hi; ++m) {
             const float dl = ((float)(m - mi) - mf) * h;
             const float d = dl * pd;
@@ -409,9 +416,11 @@ __global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int s
 #pragma unroll
             for (int s = 0; s < S; ++s) acc[s] = fmaf(w, q[s], acc[s]);
         }
+#pragma unroll
+        for (int s = 0; s < S; ++s) accd[s] += (double)acc[s];
     }
 #pragma unroll
-    for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = acc[s];
+    for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = (float)accd[s];
 }
 
 BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img)

@@ -17,7 +17,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libfbct_oracle.so")
+# FBCT_ORACLE_LIBRARY names another build of the oracle (tests/run_sanitizers.py).
+_LIB_PATH = os.environ.get("FBCT_ORACLE_LIBRARY") or os.path.join(_HERE, "_build", "libfbct_oracle.so")
 _lib = None
 
 
@@ -47,6 +48,8 @@ class OrcCone(ctypes.Structure):
 
 def build() -> str:
     """Compile the oracle with its Makefile (gcc only)."""
+    if os.environ.get("FBCT_ORACLE_LIBRARY"):
+        return _LIB_PATH
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
 
