@@ -116,6 +116,7 @@ SIGNATURES = [
     ("fbct_fbp_workspace_size", ctypes.c_size_t, [_P, _I]),
     ("fbct_fbp", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fbp_ex", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, ctypes.c_uint, _P]),
+    ("fbct_fbp_noalloc", _I, [_P, _P, _P, _I, _P]),
     ("fbct_fbp_adjoint", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_fdk", _I, [_P, _P, _P, _I, _P, ctypes.c_size_t, _P]),
     ("fbct_batch_create", _I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P)]),

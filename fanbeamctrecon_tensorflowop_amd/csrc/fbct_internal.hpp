@@ -296,6 +296,8 @@ bool parker_weights(const std::vector<float> &angles, const std::vector<double> 
 
 // Get (uploading if needed) the tables for the current device.
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out);
+// The tables on the current device if they are resident (never uploads or allocates).
+fbct_status resident_tables(fbct_plan *plan, const DeviceTables **out);
 // The plan's Parker weights on the current device (computed and uploaded on first use).
 fbct_status device_parker(fbct_plan *plan, const DeviceTables *t, const float **out);
 // cos(gamma_c) per detector column (the backprojection's column map), uploaded on first use.

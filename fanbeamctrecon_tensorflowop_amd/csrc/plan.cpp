@@ -611,6 +611,21 @@ fbct_status device_cosine(fbct_plan *plan, const DeviceTables *t, const float **
     return FBCT_OK;
 }
 
+fbct_status resident_tables(fbct_plan *plan, const DeviceTables **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    std::lock_guard<std::mutex> lk(plan->mu);
+    for (const DeviceTables &t : plan->dev)
+        if (t.device == dev) {
+            *out = &t;
+            return FBCT_OK;
+        }
+    return fail(FBCT_ERR_INVALID_ARGUMENT, "plan tables are not resident on device %d: call fbct_plan_upload first",
+                dev);
+}
+
 fbct_status device_tables(fbct_plan *plan, const DeviceTables **out)
 {
     int dev = -1;

@@ -237,6 +237,19 @@ fbct_status fbct_fbp_ex(fbct_plan *plan, const float *sino, float *image, int n_
 fbct_status fbct_fbp_short_scan(fbct_plan *plan, const float *sino, float *image, int n_slices,
                                 void *workspace, size_t workspace_bytes, void *stream);
 
+/* fbct_fbp without a workspace argument, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The filtered
+ * sinograms go to a device array of FBCT_FBP_NOALLOC_SCRATCH_BYTES (64 MiB) that the
+ * library holds on each GPU it runs on until the process ends; slices are processed in
+ * chunks that fit it (16 clinical 888 x 984 slices per chunk), and a plan whose single
+ * slice needs more is refused with FBCT_ERR_WORKSPACE.  The plan's tables must already be
+ * resident on the device (fbct_plan_upload or an earlier execute call on it), else
+ * FBCT_ERR_INVALID_ARGUMENT.  Results equal fbct_fbp's.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (different streams or threads) race. */
+#define FBCT_FBP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+fbct_status fbct_fbp_noalloc(fbct_plan *plan, const float *sino, float *image, int n_slices, void *stream);
+
 /* The reference FDK op ("BackProjection", FDK/FBP.cc:16-155 -> BP_One_Rot_3D_GPU,
  * FDK/GPURecon_GPU.cu:163-268; kernel :15-50).  The reference pins nZ = nRows = 1
  * (FDK/FBP.cc:85,98), which makes it the FBP of each slice; unlike the reference
