@@ -19,6 +19,8 @@ REPO = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD_DIR = os.path.join(PKG_DIR, "_build")
 LIB_PATH = os.path.join(PKG_DIR, "libfbct.so")
+PROBE_SRC = os.path.join(PKG_DIR, "probes", "probes.hip")
+PROBE_LIB_PATH = os.path.join(PKG_DIR, "libfbct_probes.so")
 ARCH = os.environ.get("FBCT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -56,21 +58,28 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 0, verbose: bool = False) -> str:
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    srcs = sources()
-    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
-    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB_PATH + ".tmp"
+def _link(objs, path: str) -> None:
+    tmp = path + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [
         "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
-    os.replace(tmp, LIB_PATH)
+    os.replace(tmp, path)
+
+
+def build(jobs: int = 0, verbose: bool = False) -> str:
+    """Build libfbct.so and the counter-calibration probes (libfbct_probes.so, measurement
+    only: bench.py's profile mode loads it, the operators never do)."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = sources() + [PROBE_SRC]
+    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    _link(objs[:-1], LIB_PATH)
+    _link(objs[-1:], PROBE_LIB_PATH)
     if verbose:
-        print("built", LIB_PATH)
+        print("built", LIB_PATH, "and", PROBE_LIB_PATH)
     return LIB_PATH
 
 

@@ -24,6 +24,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "fbct_internal.hpp"
 
 namespace fbct {
@@ -184,9 +186,10 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
     static_assert(S % 4 == 0, "a staged column is read four slices at a time");
     constexpr int VB = kBpStageVB, SQ = S / 4;
     __shared__ BpStage<S> st;
-    unsigned t, g;  // XCD-aware (tile, slice group)
-    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
-    const int tx0 = (int)(t % gridDim.x) * kBpTx, ty0 = (int)(t / gridDim.x) * kBpTy;
+    unsigned tbx, tby, g;  // blocked, XCD-aware order (neighbouring tiles share staged columns in L2)
+    xcd_blocked_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x, gridDim.y, gridDim.z,
+                     tbx, tby, g);
+    const int tx0 = (int)tbx * kBpTx, ty0 = (int)tby * kBpTy;
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     // wave w takes tile rows 4w..4w+3; each 16 lanes of it a 4x4 pixel block, whose
@@ -283,24 +286,33 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
 // ------------------------------------------------------------------ transpose
 // BP^T: sino[v][c] = sum over FOV pixels p of img[p] w_p hat_c(xi_p), the transpose of
 // the two-tap interpolation above (clamped taps folded into the edge columns, the
-// 0 < xi < nCols cut kept; w_p = 1 or 1/(U^2+t^2)), gathered by one thread per (view,
-// column), so every sum has a fixed order and no atomics are needed.
+// 0 < xi < nCols cut kept; w_p = 1 or 1/(U^2+t^2)), gathered per (view, column), so every
+// sum has a fixed order and no atomics are needed.
 //
 // Column c's taps come from the pixels whose column coordinate lies within one column of
-// its node xi = c + 1/2, i.e. between the node rays of columns c - 1 and c + 1.  The
-// thread marches along the axis its node ray runs closest to (theta = beta + gamma_c; y if
-// |sin theta| >= |cos theta|, else x).  On line k of that axis the node ray crosses the
-// other axis at pixel coordinate m*(k) = a k + b (one FP64 FMA), and the candidates are
-// the pixels within hw(k) = L*(k) tan|dCol| / ((|pd| - |pl| tan|dCol|) h) of m*(k), cut to
-// the line's FOV range.  Each candidate's position relative to the node ray comes from
-// its small offset Delta = (m - m*(k)) h along the line: perpendicular distance
-// d = Delta pd, distance from the source L = L*(k) + Delta pl, and
-// (xi - 1/2) - c = atan(d/L) / dCol = z (1 - z^2/3) / dCol, z = d/L (|z| is at most about
-// tan|dCol|, so the next term is below 1e-9 column).  Every quantity is small, so the
-// column offset is accurate to a few 1e-7 column, where an FP32 column coordinate carries
-// half an ulp of nCols (3e-5 column at 888 columns); that error, amplified by the ramp
-// filter that follows in FBP^T, is what a scatter with the forward kernels' coordinate
-// would leave.
+// its node xi = c + 1/2, i.e. between the node rays of columns c - 1 and c + 1.  A node
+// ray (theta = beta + gamma_c) is marched along image lines: rows or columns of pixels.
+// On line k the ray crosses the other axis at pixel coordinate m*(k) = a k + b (one FP64
+// FMA), and the candidates are the pixels within hw(k) = L*(k) tan|dCol| /
+// ((|pd| - |pl| tan|dCol|) h) of m*(k), cut to the line's FOV range.  Each candidate's
+// position relative to the node ray comes from its small offset Delta = (m - m*(k)) h
+// along the line: perpendicular distance d = Delta pd, distance from the source
+// L = L*(k) + Delta pl, and (xi - 1/2) - c = atan(d/L) / dCol = z (1 - z^2/3) / dCol,
+// z = d/L (|z| is at most about tan|dCol|, so the next term is below 1e-9 column).  Every
+// quantity is small, so the column offset is accurate to a few 1e-7 column, where an FP32
+// column coordinate carries half an ulp of nCols (3e-5 column at 888 columns); that
+// error, amplified about 70-fold by the ramp filter that follows in FBP^T, is what a
+// scatter with the forward kernels' coordinate would leave.  Sums are FP32 over the lines
+// of a chunk and FP64 across chunks, for the same reason.
+//
+// Work: a workgroup takes 64 adjacent columns (one per lane) of 16 adjacent views (4 per
+// thread; wave w holds views w, w + 4, ...) and S slices.  Its rays march a common axis
+// (the one its middle ray runs closest to) in chunks of 8 lines; per chunk the workgroup
+// stages the pixels its candidates span -- the union of every ray's candidate range over
+// the chunk's lines, each bound linear in k and so extreme at a chunk end -- for its S
+// slices in LDS, and each candidate is one LDS read of S slices.  Each pixel staged serves
+// about two columns of each of the 16 views.  A ray more than 60 degrees off that axis
+// (wide fans with few columns) is gathered on its own axis from global memory.
 struct BpaArgs {
     const float *img;
     float *sino;
@@ -313,47 +325,54 @@ struct BpaArgs {
     float rFOV, tanc, icol;  // rFOV, tan|dCol|, 1/dCol
 };
 
-constexpr int kBpaViews = 4;  // views per workgroup (one per wave), 64 columns per wave
+constexpr int kBpaVpt = 4;                   // views per thread
+constexpr int kBpaViewsWg = 4 * kBpaVpt;     // views per workgroup
+constexpr int kBpaChunk = 8;                 // lines per staged chunk
+constexpr int kBpaWin = 160;                 // staged pixels per line (wider windows read global memory)
 
-template <bool kWeighted, int S>
-__global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int slice0)
+template <int S> struct BpaTap;
+template <> struct BpaTap<1> { using T = float; };
+template <> struct BpaTap<2> { using T = float2; };
+template <> struct BpaTap<4> { using T = float4; };
+
+template <int S>
+__device__ __forceinline__ float tap_at(const typename BpaTap<S>::T &t, int s)
 {
-    unsigned t, g;  // XCD-aware (tile, slice group)
-    xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x * gridDim.y, gridDim.z, t, g);
-    const int c = (int)(t % gridDim.x) * 64 + ((int)threadIdx.x & 63);
-    const int v = (int)(t / gridDim.x) * kBpaViews + ((int)threadIdx.x >> 6);
-    if (c >= A.nCols || v >= A.nViews) return;  // no barriers below
-    const size_t sl = (size_t)slice0 + (size_t)S * g;
+    if constexpr (S == 1) return t;
+    else return reinterpret_cast<const float *>(&t)[s];
+}
 
-    // node ray of column c in view v: d(x, y) = x sin th - y cos th - R sin gam,
-    // L(x, y) = R cos gam - x cos th - y sin th
-    const double2 vb = A.csd[v], cg = A.col_csd[c];
-    const double cth = vb.x * cg.x - vb.y * cg.y, sth = vb.y * cg.x + vb.x * cg.y;
-    const double Rs = A.R * cg.y, Rc = A.R * cg.x;
-    const bool along_y = fabs(sth) >= fabs(cth);
-    double a, b, L0, Ls, p0, ext;
-    float pd, pl, h;
-    int n_major, s_major, s_minor;
-    const int2 *fov;
+// Node ray of one (view, column) on a marching axis.
+struct BpaRay {
+    double a, b;               // m*(k) = a k + b
+    float L0, Ls, pd, pl, hwc;
+    int klo, khi;              // lines holding candidates (klo > khi: none)
+};
+
+__device__ __forceinline__ void bpa_ray(const BpaArgs &A, double cth, double sth, double Rs, double Rc, bool along_y,
+                                        BpaRay &r)
+{
+    double p0, ext;
+    float h;
     if (along_y) {  // lines are image rows j; candidates along x
         const double cot = cth / sth, y0 = A.ymin + 0.5 * A.dy, xs0 = cot * y0 + Rs / sth;
-        a = cot * A.dy / A.dx;
-        b = (xs0 - A.xmin) / A.dx - 0.5;
-        L0 = Rc - cth * xs0 - sth * y0;
-        Ls = -A.dy / sth;
-        pd = (float)sth, pl = (float)(-cth), h = (float)A.dx;
-        n_major = A.nY, s_major = A.nX, s_minor = 1, fov = A.fov_row;
+        r.a = cot * A.dy / A.dx;
+        r.b = (xs0 - A.xmin) / A.dx - 0.5;
+        r.L0 = (float)(Rc - cth * xs0 - sth * y0);
+        r.Ls = (float)(-A.dy / sth);
+        r.pd = (float)sth, r.pl = (float)(-cth), h = (float)A.dx;
         p0 = -Rs * cth, ext = fabs(sth);
     } else {  // lines are image columns i; candidates along y
         const double tn = sth / cth, x0 = A.xmin + 0.5 * A.dx, ys0 = tn * x0 - Rs / cth;
-        a = tn * A.dx / A.dy;
-        b = (ys0 - A.ymin) / A.dy - 0.5;
-        L0 = Rc - cth * x0 - sth * ys0;
-        Ls = -A.dx / cth;
-        pd = (float)(-cth), pl = (float)(-sth), h = (float)A.dy;
-        n_major = A.nX, s_major = 1, s_minor = A.nX, fov = A.fov_col;
+        r.a = tn * A.dx / A.dy;
+        r.b = (ys0 - A.ymin) / A.dy - 0.5;
+        r.L0 = (float)(Rc - cth * x0 - sth * ys0);
+        r.Ls = (float)(-A.dx / cth);
+        r.pd = (float)(-cth), r.pl = (float)(-sth), h = (float)A.dy;
         p0 = Rs * sth, ext = fabs(cth);
     }
+    const float apd = fabsf(r.pd) - fabsf(r.pl) * A.tanc;
+    r.hwc = apd > 0.05f ? A.tanc / (apd * h) * 1.0001f : 1e9f;
     // lines holding candidates: points within delta of the node ray (delta bounds every
     // candidate's |d|) and inside the FOV disk have |s| <= H along the ray from its foot
     // Rs (sin th, -cos th); their coordinate on the marching axis is p0 + s dir +- delta
@@ -361,66 +380,229 @@ __global__ __launch_bounds__(256) void bp_adjoint_gather_kernel(BpaArgs A, int s
     const double dn = fmax(fabs(Rs) - delta, 0.0);
     const double H2 = (double)A.rFOV * A.rFOV - dn * dn;
     if (!(H2 > 0.0)) {
-        for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = 0.0f;
+        r.klo = 1, r.khi = 0;
         return;
     }
     const double span = sqrt(H2) * ext + delta * (along_y ? fabs(cth) : fabs(sth));
     const double org = along_y ? A.ymin : A.xmin, step = along_y ? A.dy : A.dx;
-    const int klo = max((int)floor((p0 - span - org) / step - 0.5) - 1, 0);
-    const int khi = min((int)ceil((p0 + span - org) / step - 0.5) + 1, n_major - 1);
+    const int n_major = along_y ? A.nY : A.nX;
+    r.klo = max((int)floor((p0 - span - org) / step - 0.5) - 1, 0);
+    r.khi = min((int)ceil((p0 + span - org) / step - 0.5) + 1, n_major - 1);
+}
 
-    const float L0f = (float)L0, Lsf = (float)Ls;
-    const float apd = fabsf(pd) - fabsf(pl) * A.tanc;
-    const float hwc = apd > 0.05f ? A.tanc / (apd * h) * 1.0001f : 1e9f;
-    const float icol = A.icol, c3 = -A.icol * (1.0f / 3.0f);
-    // hat of column c in rel = (xi - 1/2) - c: 1 - |rel| on (-1, 1); the edge columns take
-    // the clamped taps (weight 1 for xi - 1/2 beyond their node) up to the cut 0 < xi < nCols
-    const float clo = c == 0 ? 0.0f : -3.0e38f, chi = c == A.nCols - 1 ? 0.0f : 3.0e38f;
-    const float cut_lo = c == 0 ? -0.5f : -1.0f, cut_hi = c == A.nCols - 1 ? 0.5f : 1.0f;
+// Hat of column c in rel = (xi - 1/2) - c: 1 - |rel| on (-1, 1); the edge columns take
+// the clamped taps (weight 1 for xi - 1/2 beyond their node) up to the cut 0 < xi < nCols.
+struct BpaHat {
+    float clo, chi, cut_lo, cut_hi, icol, c3;
+};
 
-    const float *img = A.img + sl * A.img_stride;
-    // FP32 sums over a line's candidates, FP64 across lines: FBP^T applies the ramp
-    // filter to this output, which amplifies its relative rounding about 70-fold on
-    // smooth images, and a plain FP32 sum over a column's ~1500 candidates would carry
-    // ~4e-7 of it
-    double accd[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) accd[s] = 0.0;
-    for (int k = klo; k <= khi; ++k) {
-        const int2 fr = fov[k];
-        const double ms = fma(a, (double)k, b);
-        const double mfl = floor(ms);
-        const int mi = (int)mfl;
-        const float mf = (float)(ms - mfl);
-        const float Lk = fmaf((float)k, Lsf, L0f);
-        const float hw = fmaf(Lk, hwc, 1.0e-3f);
-        const int lo = max(fr.x, mi + (int)ceilf(mf - hw));
-        const int hi = min(fr.y, mi + (int)floorf(mf + hw));
-        const int base = k * s_major;
-        float acc[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) acc[s] = 0.0f;
-        for (int m = lo; m <= hi; ++m) {
-            const float dl = ((float)(m - mi) - mf) * h;
-            const float d = dl * pd;
-            const float L = fmaf(dl, pl, Lk);
-            const float z = d * __builtin_amdgcn_rcpf(L);
-            const float rel = z * fmaf(z * z, c3, icol);
-            float w = 1.0f - fabsf(fminf(fmaxf(rel, clo), chi));
-            w = (rel > cut_lo && rel < cut_hi) ? w : 0.0f;
-            if constexpr (kWeighted) w *= __builtin_amdgcn_rcpf(fmaf(L, L, d * d));
-            const float *px = img + base + m * s_minor;
-            float q[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) q[s] = px[(size_t)s * A.img_stride];
-#pragma unroll
-            for (int s = 0; s < S; ++s) acc[s] = fmaf(w, q[s], acc[s]);
-        }
-#pragma unroll
-        for (int s = 0; s < S; ++s) accd[s] += (double)acc[s];
+// Calls fn(m, w) for the candidates m of line k (FOV range fr) of ray r, w the hat weight
+// (times 1/L^2 for the weighted form); n_minor bounds the candidate range.
+template <bool kWeighted, typename Fn>
+__device__ __forceinline__ void bpa_line(const BpaRay &r, const BpaHat &hh, float h, int k, int2 fr, int n_minor, Fn &&fn)
+{
+    const double ms = fma(r.a, (double)k, r.b);
+    const double mfl = floor(ms);
+    const int mi = (int)mfl;
+    const float mf = (float)(ms - mfl);
+    const float Lk = fmaf((float)k, r.Ls, r.L0);
+    const float hw = fminf(fmaf(Lk, r.hwc, 1.0e-3f), (float)(2 * n_minor + 2));
+    const int lo = max(fr.x, mi + (int)ceilf(mf - hw));
+    const int hi = min(fr.y, mi + (int)floorf(mf + hw));
+    for (int m = lo; m <= hi; ++m) {
+        const float dl = ((float)(m - mi) - mf) * h;
+        const float d = dl * r.pd;
+        const float L = fmaf(dl, r.pl, Lk);
+        const float z = d * __builtin_amdgcn_rcpf(L);
+        const float rel = z * fmaf(z * z, hh.c3, hh.icol);
+        float w = 1.0f - fabsf(fminf(fmaxf(rel, hh.clo), hh.chi));
+        w = (rel > hh.cut_lo && rel < hh.cut_hi) ? w : 0.0f;
+        if constexpr (kWeighted) w *= __builtin_amdgcn_rcpf(fmaf(L, L, d * d));
+        fn(m, w);
     }
+}
+
+__device__ __forceinline__ int wave_min(int x)
+{
 #pragma unroll
-    for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = (float)accd[s];
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+    return x;
+}
+
+__device__ __forceinline__ int wave_max(int x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+    return x;
+}
+
+// Slices [slice0 + S*g, +S) of the batch, g = the workgroup's slice group.
+template <bool kWeighted, int S>
+__global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int slice0)
+{
+    using TapT = typename BpaTap<S>::T;
+    __shared__ TapT tile[kBpaChunk * kBpaWin];
+    __shared__ int red[2][4];
+    unsigned cb, vg, g;  // blocked, XCD-aware order over (column block, view group, slice group)
+    xcd_blocked_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x, gridDim.y, gridDim.z,
+                     cb, vg, g);
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int c = (int)cb * 64 + lane;
+    const bool cvalid = c < A.nCols;
+    const int cc = min(c, A.nCols - 1);
+    const size_t sl = (size_t)slice0 + (size_t)S * g;
+    const float *img = A.img + sl * A.img_stride;
+
+    // the workgroup's marching axis: the one its middle ray runs closest to
+    bool along_y;
+    {
+        const double2 vm = A.csd[min((int)vg * kBpaViewsWg + kBpaViewsWg / 2, A.nViews - 1)];
+        const double2 cm = A.col_csd[min((int)cb * 64 + 32, A.nCols - 1)];
+        along_y = fabs(vm.y * cm.x + vm.x * cm.y) >= fabs(vm.x * cm.x - vm.y * cm.y);
+    }
+    const double2 cg = A.col_csd[cc];
+    const double Rs = A.R * cg.y, Rc = A.R * cg.x;
+    BpaHat hh;
+    hh.clo = c == 0 ? 0.0f : -3.0e38f;
+    hh.chi = c == A.nCols - 1 ? 0.0f : 3.0e38f;
+    hh.cut_lo = c == 0 ? -0.5f : -1.0f;
+    hh.cut_hi = c == A.nCols - 1 ? 0.5f : 1.0f;
+    hh.icol = A.icol;
+    hh.c3 = -A.icol * (1.0f / 3.0f);
+
+    BpaRay r[kBpaVpt];
+    bool own[kBpaVpt];  // marched on its own axis (far from the workgroup's)
+    int kmin = INT_MAX, kmax = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < kBpaVpt; ++j) {
+        const int v = (int)vg * kBpaViewsWg + wave + 4 * j;
+        const double2 vb = A.csd[min(v, A.nViews - 1)];
+        const double cth = vb.x * cg.x - vb.y * cg.y, sth = vb.y * cg.x + vb.x * cg.y;
+        own[j] = (along_y ? fabs(sth) : fabs(cth)) < 0.5;
+        bpa_ray(A, cth, sth, Rs, Rc, own[j] ? fabs(sth) >= fabs(cth) : along_y, r[j]);
+        if (!cvalid || v >= A.nViews) r[j].klo = 1, r[j].khi = 0;
+        if (!own[j] && r[j].klo <= r[j].khi) kmin = min(kmin, r[j].klo), kmax = max(kmax, r[j].khi);
+    }
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    if (lane == 0) red[0][wave] = kmin, red[1][wave] = kmax;
+    __syncthreads();
+    const int Klo = min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3]));
+    const int Khi = max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3]));
+    __syncthreads();  // red is rewritten per chunk
+
+    const int s_major = along_y ? A.nX : 1, s_minor = along_y ? 1 : A.nX;
+    const int n_minor = along_y ? A.nX : A.nY;
+    const int2 *fov = along_y ? A.fov_row : A.fov_col;
+    const float h = along_y ? (float)A.dx : (float)A.dy;
+    auto load_taps = [&](size_t off) {
+        TapT t;
+        float *tp = reinterpret_cast<float *>(&t);
+#pragma unroll
+        for (int s = 0; s < S; ++s) tp[s] = img[off + (size_t)s * A.img_stride];
+        return t;
+    };
+    double accd[kBpaVpt][S];
+#pragma unroll
+    for (int j = 0; j < kBpaVpt; ++j)
+#pragma unroll
+        for (int s = 0; s < S; ++s) accd[j][s] = 0.0;
+
+    for (int k0 = Klo; k0 <= Khi; k0 += kBpaChunk) {
+        const int k1 = min(k0 + kBpaChunk - 1, Khi);
+        // this thread's candidate range over the chunk (bounds linear in k: ends suffice)
+        int wlo = INT_MAX, whi = INT_MIN;
+#pragma unroll
+        for (int j = 0; j < kBpaVpt; ++j) {
+            const int ka = max(k0, r[j].klo), kb = min(k1, r[j].khi);
+            if (own[j] || ka > kb) continue;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int ke = e ? kb : ka;
+                const double ms = fma(r[j].a, (double)ke, r[j].b);
+                const float hw = fminf(fmaf(fmaf((float)ke, r[j].Ls, r[j].L0), r[j].hwc, 1.0e-3f), (float)(2 * n_minor + 2));
+                wlo = min(wlo, (int)floor(ms - (double)hw));
+                whi = max(whi, (int)ceil(ms + (double)hw));
+            }
+        }
+        wlo = wave_min(wlo);
+        whi = wave_max(whi);
+        if (lane == 0) red[0][wave] = wlo, red[1][wave] = whi;
+        __syncthreads();  // also: the previous chunk's reads of the tile are done
+        const int mlo = max(min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3])) - 1, 0);
+        const int mhi = min(min(max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3])) + 1, n_minor - 1),
+                            mlo + kBpaWin - 1);
+        const int nw = max(mhi - mlo + 1, 0);  // 0: no candidates in this chunk
+        const int nl = k1 - k0 + 1;
+        if (nw > 0) {
+            // rows: consecutive threads along the row; columns: along the chunk's lines,
+            // which are consecutive pixels of an image row
+            for (int e = (int)threadIdx.x; e < nl * nw; e += 256) {
+                int li, mm;
+                if (along_y) li = e / nw, mm = e - li * nw;
+                else mm = e / nl, li = e - mm * nl;
+                tile[li * kBpaWin + mm] = load_taps((size_t)(k0 + li) * s_major + (size_t)(mlo + mm) * s_minor);
+            }
+        }
+        __syncthreads();  // tile ready
+#pragma unroll
+        for (int j = 0; j < kBpaVpt; ++j) {
+            if (own[j]) continue;
+            float lacc[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
+            for (int k = max(k0, r[j].klo); k <= min(k1, r[j].khi); ++k) {
+                const int base = (k - k0) * kBpaWin - mlo;
+                bpa_line<kWeighted>(r[j], hh, h, k, fov[k], n_minor, [&](int m, float w) {
+                    const TapT t = m >= mlo && m - mlo < nw
+                                       ? tile[base + m]
+                                       : load_taps((size_t)k * s_major + (size_t)m * s_minor);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                });
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
+        }
+    }
+
+    // rays far from the workgroup's axis: their own axis, taps from global memory, FP32
+    // sums over 8 lines and FP64 across them
+#pragma unroll
+    for (int j = 0; j < kBpaVpt; ++j) {
+        if (!own[j] || r[j].klo > r[j].khi) continue;
+        const int v = (int)vg * kBpaViewsWg + wave + 4 * j;
+        const double2 vb = A.csd[v];
+        const double sth = vb.y * cg.x + vb.x * cg.y, cth = vb.x * cg.x - vb.y * cg.y;
+        const bool ay = fabs(sth) >= fabs(cth);
+        const int sj = ay ? A.nX : 1, si = ay ? 1 : A.nX;
+        const int nm = ay ? A.nX : A.nY;
+        const int2 *fv = ay ? A.fov_row : A.fov_col;
+        const float hj = ay ? (float)A.dx : (float)A.dy;
+        for (int k0 = r[j].klo; k0 <= r[j].khi; k0 += kBpaChunk) {
+            float lacc[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
+            for (int k = k0; k <= min(k0 + kBpaChunk - 1, r[j].khi); ++k)
+                bpa_line<kWeighted>(r[j], hh, hj, k, fv[k], nm, [&](int m, float w) {
+                    const TapT t = load_taps((size_t)k * sj + (size_t)m * si);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                });
+#pragma unroll
+            for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
+        }
+    }
+
+    if (!cvalid) return;
+#pragma unroll
+    for (int j = 0; j < kBpaVpt; ++j) {
+        const int v = (int)vg * kBpaViewsWg + wave + 4 * j;
+        if (v >= A.nViews) continue;
+#pragma unroll
+        for (int s = 0; s < S; ++s) A.sino[(sl + s) * A.sino_stride + (size_t)v * A.nCols + c] = (float)accd[j][s];
+    }
 }
 
 BpArgs make_args(const fbct_plan &p, const DeviceTables &t, const float *sino, float *img)
@@ -520,18 +702,15 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     A.rFOV = p.g.rFOV;
     A.tanc = p.d.bpa_tan;
     A.icol = p.d.bp_xi_scale;
-    // slice groups of 16/8/4/2 while they leave enough workgroups to fill the chip
-    const dim3 tiles((p.g.nCols + 63) / 64, (p.g.nViews + kBpaViews - 1) / kBpaViews);
-    const long long nt = (long long)tiles.x * tiles.y;
+    // groups of 4 slices, the remainder in groups of 2 and 1 (the same arithmetic per slice)
+    const dim3 tiles((p.g.nCols + 63) / 64, (p.g.nViews + kBpaViewsWg - 1) / kBpaViewsWg);
     int done = 0;
-    for (int S : {16, 8, 4, 2, 1}) {
+    for (int S : {4, 2, 1}) {
         const int groups = (n - done) / S;
-        if (groups == 0 || (S > 1 && nt * groups < 2048)) continue;
+        if (groups == 0) continue;
         const dim3 grid(tiles.x, tiles.y, groups);
-#define FBCT_BPA(W, SS) hipLaunchKernelGGL((bp_adjoint_gather_kernel<W, SS>), grid, dim3(256), 0, stream, A, done)
+#define FBCT_BPA(W, SS) hipLaunchKernelGGL((bp_adjoint_staged_kernel<W, SS>), grid, dim3(256), 0, stream, A, done)
         switch (S) {
-        case 16: if (weighted) FBCT_BPA(true, 16); else FBCT_BPA(false, 16); break;
-        case 8: if (weighted) FBCT_BPA(true, 8); else FBCT_BPA(false, 8); break;
         case 4: if (weighted) FBCT_BPA(true, 4); else FBCT_BPA(false, 4); break;
         case 2: if (weighted) FBCT_BPA(true, 2); else FBCT_BPA(false, 2); break;
         default: if (weighted) FBCT_BPA(true, 1); else FBCT_BPA(false, 1); break;
@@ -541,7 +720,7 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
         if (done == n) break;
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_adjoint_gather_kernel launch: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "bp_adjoint_staged_kernel launch: %s", hipGetErrorString(e));
     return FBCT_OK;
 }
 

@@ -51,6 +51,31 @@ __host__ __device__ inline void xcd_tile(unsigned b, unsigned tiles, unsigned gr
     tile = l % tiles;
 }
 
+// (tile x, tile y, slice group) of workgroup b in a grid of gx x gy tiles and `groups`
+// slice groups: each XCD takes a contiguous range of the logical order, which runs over
+// the slice groups, in each over blocks of kB x kB tiles (block rows top to bottom,
+// blocks left to right, partial blocks at the right and bottom edges) and in a block
+// over its tiles row by row.  The workgroups resident on one XCD at a time then cover a
+// compact square of the image, whose detector footprints in every view overlap, so the
+// columns they stage are shared through that XCD's L2.
+__host__ __device__ inline void xcd_blocked_tile(unsigned b, unsigned gx, unsigned gy, unsigned groups, unsigned &tx,
+                                                 unsigned &ty, unsigned &group)
+{
+    constexpr unsigned kB = 8;
+    const unsigned tiles = gx * gy;
+    const unsigned l = xcd_block(b, tiles * groups);
+    group = l / tiles;
+    const unsigned t = l % tiles;
+    const unsigned br = t / (gx * kB);              // block row (full rows hold gx * kB tiles)
+    const unsigned h = gy - br * kB < kB ? gy - br * kB : kB;  // its height
+    const unsigned rem = t - br * gx * kB;
+    const unsigned bc = rem / (kB * h);             // block in the row (full blocks hold kB * h tiles)
+    const unsigned w = gx - bc * kB < kB ? gx - bc * kB : kB;  // its width
+    const unsigned r = rem - bc * kB * h;
+    ty = br * kB + r / w;
+    tx = bc * kB + r % w;
+}
+
 // BP atan(z) = z + z^3 R(z^2): at most this many coefficients of R.
 constexpr int kBpAtanMax = 9;
 

@@ -702,7 +702,10 @@ template <int S, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_staged_kernel(FpaArgs A)
 {
     __shared__ CellShared<S> sh;
-    cell_staged_tile<S, TI, TO>(A, sh, blockIdx.x, blockIdx.y, blockIdx.z * S);
+    unsigned tx, ty, g;  // blocked, XCD-aware order (neighbouring tiles share staged columns in L2)
+    xcd_blocked_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x, gridDim.y, gridDim.z,
+                     tx, ty, g);
+    cell_staged_tile<S, TI, TO>(A, sh, (int)tx, (int)ty, (int)g * S);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, tile), items
