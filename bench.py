@@ -150,36 +150,49 @@ COUNTER_PASSES = (
 )
 
 
+def _clinical_ops(torch, slices):
+    c = clinical_setup(torch, slices)
+    plan, x, xb, target = c["plan"], c["x"], c["xb"], c["target"]
+    rb = (plan.fp(xb) - target).to(torch.bfloat16)
+    sino = target.contiguous()
+    return [("clinical.fp_step", lambda: plan.fp_residual(xb, target, out_dtype=torch.bfloat16)),
+            ("clinical.fp_adjoint_step", lambda: plan.fp_adjoint(rb)),
+            ("clinical.fp", lambda: plan.fp(x)),
+            ("clinical.fp_adjoint", lambda: plan.fp_adjoint(sino)),
+            ("clinical.bp", lambda: plan.bp(sino)),
+            ("clinical.fbp", lambda: plan.fbp(sino)),
+            ("clinical.bp_adjoint", lambda: plan.bp_adjoint(x)),
+            ("clinical.fbp_adjoint", lambda: plan.fbp_adjoint(x))]
+
+
+def _uneven_ops(torch):
+    u = uneven_setup(torch)
+    batch, x, xb, target = u["batch"], u["x"], u["xb"], u["target"]
+    rb = (batch.fp(xb) - target).to(torch.bfloat16)
+    yo = torch.empty(batch.sino_numel, dtype=torch.float32, device=x.device)
+    go = torch.empty(batch.image_numel, dtype=torch.float32, device=x.device)
+    return [("uneven.fp_step", lambda: batch.fp(xb, out=yo)),
+            ("uneven.fp_adjoint_step", lambda: batch.fp_adjoint(rb, out=go))]
+
+
+def _fdk_ops(torch):
+    f = fdk_setup(torch)
+    plan, proj_b = f["plan"], f["proj_b"]
+    q = plan.filter(proj_b[None])
+    return [("fdk.filter", lambda: plan.filter(proj_b[None])),
+            ("fdk.backproject", lambda: plan.backproject(q))]
+
+
 def profile_ops(torch, workload, slices):
-    """[(name, fn)] of the operations a profile dispatches once each."""
+    """[(name, fn)] of the operations a profile dispatches once each (each workload's
+    closures in a scope of their own)."""
     ops = []
     if workload in ("all", "clinical"):
-        c = clinical_setup(torch, slices)
-        plan, x, xb, target = c["plan"], c["x"], c["xb"], c["target"]
-        rb = (plan.fp(xb) - target).to(torch.bfloat16)
-        sino = target.contiguous()
-        ops += [("clinical.fp_step", lambda: plan.fp_residual(xb, target, out_dtype=torch.bfloat16)),
-                ("clinical.fp_adjoint_step", lambda: plan.fp_adjoint(rb)),
-                ("clinical.fp", lambda: plan.fp(x)),
-                ("clinical.fp_adjoint", lambda: plan.fp_adjoint(sino)),
-                ("clinical.bp", lambda: plan.bp(sino)),
-                ("clinical.fbp", lambda: plan.fbp(sino)),
-                ("clinical.bp_adjoint", lambda: plan.bp_adjoint(x)),
-                ("clinical.fbp_adjoint", lambda: plan.fbp_adjoint(x))]
+        ops += _clinical_ops(torch, slices)
     if workload in ("all", "uneven"):
-        u = uneven_setup(torch)
-        batch, x, xb, target = u["batch"], u["x"], u["xb"], u["target"]
-        rb = (batch.fp(xb) - target).to(torch.bfloat16)
-        yo = torch.empty(batch.sino_numel, dtype=torch.float32, device=x.device)
-        go = torch.empty(batch.image_numel, dtype=torch.float32, device=x.device)
-        ops += [("uneven.fp_step", lambda: batch.fp(xb, out=yo)),
-                ("uneven.fp_adjoint_step", lambda: batch.fp_adjoint(rb, out=go))]
+        ops += _uneven_ops(torch)
     if workload in ("all", "fdk"):
-        f = fdk_setup(torch)
-        plan, proj_b = f["plan"], f["proj_b"]
-        q = plan.filter(proj_b[None])
-        ops += [("fdk.filter", lambda: plan.filter(proj_b[None])),
-                ("fdk.backproject", lambda: plan.backproject(q))]
+        ops += _fdk_ops(torch)
     return ops
 
 

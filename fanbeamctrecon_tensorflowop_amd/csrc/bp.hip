@@ -325,15 +325,18 @@ struct BpaArgs {
     float rFOV, tanc, icol;  // rFOV, tan|dCol|, 1/dCol
 };
 
-constexpr int kBpaVpt = 4;                   // views per thread
-constexpr int kBpaViewsWg = 4 * kBpaVpt;     // views per workgroup
 constexpr int kBpaChunk = 8;                 // lines per staged chunk
+constexpr int kBpaS = 4, kBpaV = 4;          // slices per workgroup, views per thread (largest groups)
 constexpr int kBpaWin = 160;                 // staged pixels per line (wider windows read global memory)
 
+struct float8_t {
+    float4 a, b;
+};
 template <int S> struct BpaTap;
 template <> struct BpaTap<1> { using T = float; };
 template <> struct BpaTap<2> { using T = float2; };
 template <> struct BpaTap<4> { using T = float4; };
+template <> struct BpaTap<8> { using T = float8_t; };
 
 template <int S>
 __device__ __forceinline__ float tap_at(const typename BpaTap<S>::T &t, int s)
@@ -437,11 +440,13 @@ __device__ __forceinline__ int wave_max(int x)
 }
 
 // Slices [slice0 + S*g, +S) of the batch, g = the workgroup's slice group.
-template <bool kWeighted, int S>
+template <bool kWeighted, int S, int kBpaVpt>
 __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int slice0)
 {
+    constexpr int kBpaViewsWg = 4 * kBpaVpt;  // views per workgroup
     using TapT = typename BpaTap<S>::T;
     __shared__ TapT tile[kBpaChunk * kBpaWin];
+    __shared__ int2 fovc[kBpaChunk];  // FOV ranges of the chunk's lines
     __shared__ int red[2][4];
     unsigned cb, vg, g;  // blocked, XCD-aware order over (column block, view group, slice group)
     xcd_blocked_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x, gridDim.y, gridDim.z,
@@ -531,10 +536,12 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
         if (lane == 0) red[0][wave] = wlo, red[1][wave] = whi;
         __syncthreads();  // also: the previous chunk's reads of the tile are done
         const int mlo = max(min(min(red[0][0], red[0][1]), min(red[0][2], red[0][3])) - 1, 0);
-        const int mhi = min(min(max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3])) + 1, n_minor - 1),
-                            mlo + kBpaWin - 1);
-        const int nw = max(mhi - mlo + 1, 0);  // 0: no candidates in this chunk
+        const int mhi_all = min(max(max(red[1][0], red[1][1]), max(red[1][2], red[1][3])) + 1, n_minor - 1);
+        // a window wider than the tile (wide fans): this chunk reads global memory
+        const bool wide = mhi_all - mlo + 1 > kBpaWin;
+        const int nw = wide ? 0 : max(mhi_all - mlo + 1, 0);  // 0: nothing staged
         const int nl = k1 - k0 + 1;
+        if (threadIdx.x < (unsigned)nl) fovc[threadIdx.x] = fov[k0 + threadIdx.x];
         if (nw > 0) {
             // rows: consecutive threads along the row; columns: along the chunk's lines,
             // which are consecutive pixels of an image row
@@ -546,24 +553,46 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
             }
         }
         __syncthreads();  // tile ready
+        // lines run uniformly over the chunk (their FOV ranges from LDS); a ray skips the
+        // lines outside its own range
+        if (!wide) {
+            if (nw == 0) continue;  // no candidates in this chunk (uniform)
 #pragma unroll
-        for (int j = 0; j < kBpaVpt; ++j) {
-            if (own[j]) continue;
-            float lacc[S];
+            for (int j = 0; j < kBpaVpt; ++j) {
+                if (own[j]) continue;
+                float lacc[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
-            for (int k = max(k0, r[j].klo); k <= min(k1, r[j].khi); ++k) {
-                const int base = (k - k0) * kBpaWin - mlo;
-                bpa_line<kWeighted>(r[j], hh, h, k, fov[k], n_minor, [&](int m, float w) {
-                    const TapT t = m >= mlo && m - mlo < nw
-                                       ? tile[base + m]
-                                       : load_taps((size_t)k * s_major + (size_t)m * s_minor);
+                for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
+                for (int k = k0; k <= k1; ++k) {
+                    if (k < r[j].klo || k > r[j].khi) continue;
+                    // every candidate lies in the window (it is the union of the candidate
+                    // ranges, one pixel wider); the clamp only keeps reads inside the tile
+                    const TapT *row = &tile[(k - k0) * kBpaWin];
+                    bpa_line<kWeighted>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
+                        const TapT t = row[min(max(m - mlo, 0), nw - 1)];
 #pragma unroll
-                    for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
-                });
+                        for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                    });
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
             }
+        } else {
 #pragma unroll
-            for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
+            for (int j = 0; j < kBpaVpt; ++j) {
+                if (own[j]) continue;
+                float lacc[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
+                for (int k = max(k0, r[j].klo); k <= min(k1, r[j].khi); ++k)
+                    bpa_line<kWeighted>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
+                        const TapT t = load_taps((size_t)k * s_major + (size_t)m * s_minor);
+#pragma unroll
+                        for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                    });
+#pragma unroll
+                for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
+            }
         }
     }
 
@@ -702,18 +731,21 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     A.rFOV = p.g.rFOV;
     A.tanc = p.d.bpa_tan;
     A.icol = p.d.bp_xi_scale;
-    // groups of 4 slices, the remainder in groups of 2 and 1 (the same arithmetic per slice)
-    const dim3 tiles((p.g.nCols + 63) / 64, (p.g.nViews + kBpaViewsWg - 1) / kBpaViewsWg);
+    // groups of kBpaS slices (kBpaV views per thread), the remainder in groups of 2 and 1
+    // with 4 views per thread (the same arithmetic per slice)
     int done = 0;
-    for (int S : {4, 2, 1}) {
+    for (int S : {kBpaS, 2, 1}) {
         const int groups = (n - done) / S;
         if (groups == 0) continue;
-        const dim3 grid(tiles.x, tiles.y, groups);
-#define FBCT_BPA(W, SS) hipLaunchKernelGGL((bp_adjoint_staged_kernel<W, SS>), grid, dim3(256), 0, stream, A, done)
-        switch (S) {
-        case 4: if (weighted) FBCT_BPA(true, 4); else FBCT_BPA(false, 4); break;
-        case 2: if (weighted) FBCT_BPA(true, 2); else FBCT_BPA(false, 2); break;
-        default: if (weighted) FBCT_BPA(true, 1); else FBCT_BPA(false, 1); break;
+        const int vpt = S == kBpaS ? kBpaV : 4;
+        const dim3 grid((p.g.nCols + 63) / 64, (p.g.nViews + 4 * vpt - 1) / (4 * vpt), groups);
+#define FBCT_BPA(W, SS, VV) hipLaunchKernelGGL((bp_adjoint_staged_kernel<W, SS, VV>), grid, dim3(256), 0, stream, A, done)
+        if (S == kBpaS) {
+            if (weighted) FBCT_BPA(true, kBpaS, kBpaV); else FBCT_BPA(false, kBpaS, kBpaV);
+        } else if (S == 2) {
+            if (weighted) FBCT_BPA(true, 2, 4); else FBCT_BPA(false, 2, 4);
+        } else {
+            if (weighted) FBCT_BPA(true, 1, 4); else FBCT_BPA(false, 1, 4);
         }
 #undef FBCT_BPA
         done += groups * S;

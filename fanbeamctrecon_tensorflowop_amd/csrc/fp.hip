@@ -117,8 +117,15 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
 
     const int tid = threadIdx.x;
     constexpr int kRays = kFpThreads / G;
-    const int lane = tid % G;
-    const int c = b * kRays + tid / G;
+    // Share-major lanes: a wave holds kRw rays; lane w of it is ray w % kRw, share
+    // w / kRw.  The 16 lanes of one LDS pass then read kRw rays at one or two shares,
+    // whose rows differ, and with the tile's row pitch of 25 taps (9 mod 16 in 16-byte
+    // bank groups) distinct rows fall on distinct bank groups; with the G shares of a ray
+    // in one pass instead, shares three taps apart in adjacent rows collide.
+    constexpr int kRw = 64 / G;
+    const int wl = tid & 63;
+    const int lane = wl / kRw;  // the ray's share
+    const int c = b * kRays + (tid >> 6) * kRw + wl % kRw;
     const bool xmajor = A.major[(size_t)v * A.nb + b] != 0;
 
     float a0 = 0.0f, m0 = 0.0f, da = 0.0f, dm = 0.0f;
@@ -182,11 +189,11 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
         const int mlo = rr.x, mhi = rr.y;
         const float tend = lane == G - 1 ? (float)(alo + kSlab) : fmaf((float)(lane + 1), kShare, (float)alo);
         const int kown = slab_boundary(a0, da, ia, N, tend);
-        // lane l - 1's end (DPP row_shr:1; lane 0 of a ray takes the carry instead), and
-        // for the next slab's lane 0 the end of lane G - 1 (swizzle lane ^ (G - 1))
-        int kstart = __builtin_amdgcn_update_dpp(0, kown, 0x111, 0xf, 0xf, false);
+        // share l - 1's end (kRw lanes below; share 0 of a ray takes the carry instead),
+        // and for the next slab's share 0 the end of share G - 1
+        int kstart = __shfl(kown, max(wl - kRw, 0));
         if (lane == 0) kstart = kcarry;
-        kcarry = __builtin_amdgcn_ds_swizzle(kown, 0x1f | ((G - 1) << 10));
+        kcarry = __shfl(kown, wl % kRw + kRw * (G - 1));
         if (mlo > mhi) continue;  // no samples of this bundle in the slab
         const int M = mhi - mlo + 1;
         // M <= A.cap for every item of a launch: the plan (and the batch) check their row
@@ -275,7 +282,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
 #pragma unroll
     for (int s = 0; s < S; ++s) {
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
+        for (int off = 32; off >= kRw; off >>= 1) accd[s] += __shfl_xor(accd[s], off);
         if (lane == 0 && c < A.nCols && s0 + s < A.nSlices) {
             const size_t o = (size_t)(s0 + s) * A.sino_stride + (size_t)v * A.nCols + c;
             const float p = (float)(accd[s] * (double)A.scale);
