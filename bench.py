@@ -146,7 +146,8 @@ COUNTER_PASSES = (
            "SQ_INSTS_LDS_LOAD_BANDWIDTH", "SQ_LDS_IDX_ACTIVE", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VALU_FLOPS_FP32",
            "GRBM_GUI_ACTIVE"]),
     ("b", ["WRITE_SIZE", "TCC_HIT", "TCC_MISS", "SQ_INSTS_LDS_STORE_BANDWIDTH", "SQ_VALU_MFMA_BUSY_CYCLES",
-           "SQ_INSTS_MFMA", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_LDS", "GRBM_GUI_ACTIVE"]),
+           "SQ_INSTS_MFMA", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_LDS", "SQ_THREAD_CYCLES_VALU",
+           "SQ_ACTIVE_INST_VALU", "GRBM_GUI_ACTIVE"]),
 )
 
 
@@ -293,7 +294,10 @@ def collect_counters(args):
             ops = _group_by_marker(_read_counter_csv(files[0]), names)
             for op, v in ops.items():
                 m = merged.setdefault(op, {"kernels": v["kernels"], "counters": {}, "ns": {}})
-                m["counters"].update(v["counters"])
+                # a counter read again in a later pass keeps its first reading; the later
+                # one is kept under name_<pass>
+                for k, x in v["counters"].items():
+                    m["counters"][k if k not in m["counters"] else "%s_%s" % (k, tag)] = x
                 m["ns"][tag] = v["ns"]
         return merged
     except (OSError, subprocess.SubprocessError, ValueError) as e:
@@ -571,6 +575,22 @@ def bench_fdk(torch, args):
 
 # --------------------------------------------------------------- rooflines
 
+def _probe_rates(prof):
+    out = {}
+    try:
+        ns = {k: prof[k]["ns"].get("a") for k in ("probe.valu", "probe.lds", "probe.copy")}
+        if ns["probe.valu"]:
+            out["fp32_fma_tflops"] = round(2.0 * 64 * PROBE_VALU_WGS * 256 / 64 * PROBE_VALU_ITERS * 16 * 8 /
+                                           ns["probe.valu"] / 1e3, 1)
+        if ns["probe.lds"]:
+            out["lds_read_tbs"] = round(16.0 * PROBE_LDS_WGS * 256 * PROBE_LDS_ITERS * 4 / ns["probe.lds"] / 1e3, 1)
+        if ns["probe.copy"]:
+            out["hbm_copy_gbs"] = round(32.0 * PROBE_COPY_VEC4 / ns["probe.copy"], 1)
+    except (KeyError, TypeError, AttributeError):
+        pass
+    return out
+
+
 def _calibration(prof):
     """Scale factors and probe readings of a counter profile, or None.  GRBM_GUI_ACTIVE
     sums the 8 XCDs' busy cycles, so per-GPU-cycle rates divide by it / 8."""
@@ -588,6 +608,9 @@ def _calibration(prof):
             # SQ_INSTS_VALU_FLOPS_FP32 counts per wave instruction: flops = counter x 64
             "flops_per_count": 2.0 * 64.0 * 2 ** 30 / pv["SQ_INSTS_VALU_FLOPS_FP32"] if pv.get(
                 "SQ_INSTS_VALU_FLOPS_FP32") else 64.0,
+            # the probes' own rates (kernel time from the first pass's trace): non-packed
+            # FP32 FMA issue, conflict-free 16-byte LDS reads, a plain HBM copy
+            "measured_peaks": _probe_rates(prof),
         }
     except (KeyError, TypeError, ZeroDivisionError):
         return None
@@ -603,7 +626,7 @@ def _busy(c):
     b = {"valu_issue": (c["SQ_ACTIVE_INST_VALU"] - c.get("SQ_ACTIVE_INST_VALU2", 0.0)) / g / 32.0,
          "lds_pipe": c["SQ_LDS_IDX_ACTIVE"] / g / 32.0}
     if c.get("SQ_INSTS_MFMA"):
-        b["mfma"] = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (g / 8.0) / 1024.0
+        b["mfma"] = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (c.get("GRBM_GUI_ACTIVE_b", g) / 8.0) / 1024.0
     return b
 
 
@@ -636,22 +659,32 @@ def roofline(kernel, ms, op, prof, cal, flops=None, lds_bytes=None, model=""):
         r.update({"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
                   "note": "no counter profile: %s" % (prof.get("error") if isinstance(prof, dict) else "disabled")})
         return r
+    # the kernels the profile saw for this operation (their launches, in order, once each)
+    names = []
+    for k in e["kernels"]:
+        k = k.replace("void ", "").replace("fbct::(anonymous namespace)::", "").replace("unsigned short", "bf16")
+        if k not in names:
+            names.append(k)
+    r["kernel"] = " + ".join(names) if names else kernel
     c = e["counters"]
     traffic = c.get("FETCH_SIZE", 0.0) * 1024.0 * cal["fetch_factor"] + c.get("WRITE_SIZE", 0.0) * 1024.0 * cal[
         "write_factor"]
     busy = _busy(c)
     busy["hbm"] = traffic / sec / 1e9 / HBM_PEAK_GBS
-    bound = max(busy, key=busy.get)
+    # the busiest resource; between equal readings the one the path's model prices
+    priced = {"valu_issue": bool(flops), "mfma": bool(flops), "lds_pipe": bool(lds_bytes), "hbm": True}
+    bound = max(busy, key=lambda k: (round(busy[k], 3), priced.get(k, False)))
+    measured_lds = 64.0 * (c.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + c.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0))
     if bound == "valu_issue" and flops:
         ach, peak, unit = flops / sec / 1e12, FP32_VALU_PEAK_TFLOPS, "TFLOP/s"
     elif bound == "mfma" and flops:
         ach, peak, unit = flops / sec / 1e12, BF16_MFMA_PEAK_TFLOPS, "TFLOP/s"
-    elif bound == "lds_pipe" and lds_bytes:
-        ach, peak, unit = lds_bytes / sec / 1e9, LDS_PEAK_GBS, "GB/s"
+    elif bound == "lds_pipe":
+        # the model's LDS bytes, else the bytes the counters saw moving
+        ach, peak, unit = (lds_bytes or measured_lds) / sec / 1e9, LDS_PEAK_GBS, "GB/s"
     else:
         ach, peak, unit = traffic / sec / 1e9, HBM_PEAK_GBS, "GB/s"
     hit, miss = c.get("TCC_HIT"), c.get("TCC_MISS")
-    measured_lds = 64.0 * (c.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + c.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0))
     r.update({
         "bound": bound, "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 4),
         "traffic": int(traffic), "hbm_gbs": round(traffic / sec / 1e9, 1),
@@ -662,6 +695,10 @@ def roofline(kernel, ms, op, prof, cal, flops=None, lds_bytes=None, model=""):
         if c.get("SQ_LDS_IDX_ACTIVE") else None,
         "valu_dual_issue_per_issue": round(c.get("SQ_ACTIVE_INST_VALU2", 0.0) / c["SQ_ACTIVE_INST_VALU"], 4)
         if c.get("SQ_ACTIVE_INST_VALU") else None,
+        # active lanes per VALU instruction (divergence): thread-cycles over wave-cycles x 64,
+        # both from the second pass
+        "valu_lane_utilisation": round(c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU_b"]), 4)
+        if c.get("SQ_THREAD_CYCLES_VALU") and c.get("SQ_ACTIVE_INST_VALU_b") else None,
         "measured": {"lds_gbs": round(measured_lds / sec / 1e9, 1),
                      "fp32_tflops": round(c.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) * cal["flops_per_count"] / sec / 1e12,
                                           3)},
@@ -712,9 +749,9 @@ def main(argv=None):
         # reads 4 taps (2 B each in BF16 storage, 4 B in FP32); the cell-walk kernels do
         # less (a texel cell's taps and weights once for its 6-8 samples)
         fpm = "8 flops and 4 taps per FP sample per slice (bilinear)"
-        rl_fp = roofline("fp_kernel<4, 8, bf16, bf16> (two launches: tile classes)", ms["fp_step"], "clinical.fp_step",
+        rl_fp = roofline("fp_kernel (two launches: tile classes)", ms["fp_step"], "clinical.fp_step",
                          prof, cal, 8 * samples * S, 8 * samples * S, fpm)
-        rl_adj = roofline("fp_adjoint_cell_staged_kernel<16, bf16, float>", ms["fp_adjoint_step"],
+        rl_adj = roofline("fp_adjoint_cell_staged_kernel", ms["fp_adjoint_step"],
                           "clinical.fp_adjoint_step", prof, cal, 8 * samples * S, 8 * samples * S, fpm)
         out["roofline"] = rl_fp if ms["fp_step"] >= ms["fp_adjoint_step"] else rl_adj
         out["roofline_fp"] = rl_fp
@@ -722,10 +759,10 @@ def main(argv=None):
         bpm = "4 flops and 2 taps of 4 B per voxel-view update (linear interpolation)"
         out["roofline_other"] = {
             k: roofline(kn, ms[k], "clinical." + k, prof, cal, 4 * U * S, 8 * U * S, bpm) for k, kn in (
-                ("bp", "bp_staged_kernel<unweighted, 16 slices>"),
-                ("fbp", "ramp_gemm_kernel + bp_staged_kernel<1/L^2, 16 slices>"),
-                ("bp_adjoint", "bp_adjoint_staged_kernel<unweighted, 4 slices>"),
-                ("fbp_adjoint", "bp_adjoint_staged_kernel<1/L^2, 4 slices> + ramp_gemm_kernel"))}
+                ("bp", "bp_staged_kernel"),
+                ("fbp", "ramp_gemm_kernel + bp_staged_kernel"),
+                ("bp_adjoint", "bp_adjoint_staged_kernel"),
+                ("fbp_adjoint", "bp_adjoint_staged_kernel + ramp_gemm_kernel"))}
         result = out
     if args.workload in ("all", "uneven"):
         out, ctx = bench_uneven(torch, args)

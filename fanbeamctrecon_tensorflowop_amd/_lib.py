@@ -58,7 +58,8 @@ class FanGeom(ctypes.Structure):
 class BatchShape(ctypes.Structure):
     """Mirror of ``fbct_batch_shape``."""
     _fields_ = [("fp_rays", ctypes.c_int32), ("fp_wgs_per_cu", ctypes.c_int32),
-                ("adj_tile_x", ctypes.c_int32), ("adj_wgs_per_cu", ctypes.c_int32)]
+                ("adj_tile_x", ctypes.c_int32), ("adj_wgs_per_cu", ctypes.c_int32),
+                ("adj_stage_views", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

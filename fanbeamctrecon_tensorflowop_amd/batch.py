@@ -25,10 +25,11 @@ import torch
 from . import _lib
 from .plan import _DTYPE, _STORAGE, FanBeamPlan
 
-__all__ = ["UnevenBatch", "StepGraph", "autotune", "FP_RAYS", "ADJ_TILE_X", "WGS_PER_CU"]
+__all__ = ["UnevenBatch", "StepGraph", "autotune", "FP_RAYS", "ADJ_TILE_X", "ADJ_STAGE_VIEWS", "WGS_PER_CU"]
 
 FP_RAYS = (64, 32)
 ADJ_TILE_X = (16, 32, 8, 64)
+ADJ_STAGE_VIEWS = (0, 4, 8)  # 0: cell tiles read their rays from global memory
 WGS_PER_CU = (0, 4, 8, 16, 32)  # 0: one workgroup per work item
 
 
@@ -242,9 +243,11 @@ def autotune(batch: UnevenBatch, images: torch.Tensor, sinos: torch.Tensor, reps
     """Pick the batch's launch shape by timing every candidate, then set it.
 
     FP: rays per workgroup in ``FP_RAYS`` x workgroups per CU in ``WGS_PER_CU``;
-    FP^T: pixel tile width in ``ADJ_TILE_X`` x workgroups per CU.  The two operators are
-    tuned independently on the given packed buffers (their storage types are part of
-    the key).  Results are cached in ``$FBCT_TUNE_CACHE`` (default
+    FP^T: pixel tile width in ``ADJ_TILE_X`` x workgroups per CU with unstaged cell
+    tiles, then, at the best tile width, views per LDS staging batch in
+    ``ADJ_STAGE_VIEWS`` x workgroups per CU (the tile width shapes the gather tiles,
+    the staging the cell tiles).  The two operators are tuned independently on the given
+    packed buffers (their storage types are part of the key).  Results are cached in ``$FBCT_TUNE_CACHE`` (default
     ``~/.cache/fbct_amd/batch_shapes.json``) by device name and geometry signature.
     Returns ``{"shape": ..., "fp_ms": ..., "adj_ms": ..., "cached": bool}``."""
     dev = images.device
@@ -270,13 +273,22 @@ def autotune(batch: UnevenBatch, images: torch.Tensor, sinos: torch.Tensor, reps
     best_adj = None
     for tx in ADJ_TILE_X:
         for w in WGS_PER_CU:
-            batch.set_shape(adj_tile_x=tx, adj_wgs_per_cu=w)
+            batch.set_shape(adj_tile_x=tx, adj_wgs_per_cu=w, adj_stage_views=0)
             t = _time(lambda: batch.fp_adjoint(sinos, out=adj_out), reps)
             if verbose:
                 print("autotune adj tile=%dx%d wgs/cu=%d: %.3f ms" % (tx, 256 // tx, w, t), flush=True)
             if best_adj is None or t < best_adj[0]:
-                best_adj = (t, tx, w)
-    shape = dict(fp_rays=best_fp[1], fp_wgs_per_cu=best_fp[2], adj_tile_x=best_adj[1], adj_wgs_per_cu=best_adj[2])
+                best_adj = (t, tx, w, 0)
+    for vb in ADJ_STAGE_VIEWS[1:]:
+        for w in WGS_PER_CU:
+            batch.set_shape(adj_tile_x=best_adj[1], adj_wgs_per_cu=w, adj_stage_views=vb)
+            t = _time(lambda: batch.fp_adjoint(sinos, out=adj_out), reps)
+            if verbose:
+                print("autotune adj stage views=%d wgs/cu=%d: %.3f ms" % (vb, w, t), flush=True)
+            if t < best_adj[0]:
+                best_adj = (t, best_adj[1], w, vb)
+    shape = dict(fp_rays=best_fp[1], fp_wgs_per_cu=best_fp[2], adj_tile_x=best_adj[1], adj_wgs_per_cu=best_adj[2],
+                 adj_stage_views=best_adj[3])
     batch.set_shape(**shape)
     result = {"shape": shape, "fp_ms": best_fp[0], "adj_ms": best_adj[0]}
     if use_cache:

@@ -124,7 +124,12 @@ fbct_status check_shape(const fbct_batch_shape &s)
         s.adj_wgs_per_cu > kMaxWgsPerCu)
         return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: workgroups per CU must be in 0..%d", kMaxWgsPerCu);
     for (int k = 0; k < kAdjNumTiles; ++k)
-        if (kAdjTileX[k] == s.adj_tile_x) return FBCT_OK;
+        if (kAdjTileX[k] == s.adj_tile_x) {
+            if (s.adj_stage_views != 0 && s.adj_stage_views != 4 && s.adj_stage_views != 8)
+                return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: adj_stage_views must be 0, 4 or 8 (got %d)",
+                            s.adj_stage_views);
+            return FBCT_OK;
+        }
     return fail(FBCT_ERR_INVALID_ARGUMENT, "batch shape: adj_tile_x must be 8, 16, 32 or 64 (got %d)", s.adj_tile_x);
 }
 
@@ -213,7 +218,7 @@ fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch 
                 if (!big[g]) order.push_back(g);
             b->fp_small_cap[k] = small_cap;
         }
-        b->shape = fbct_batch_shape{64, 0, 16, 16};
+        b->shape = fbct_batch_shape{64, 0, 16, 16, 0};
     } catch (const std::bad_alloc &) {
         delete b;
         return fail(FBCT_ERR_OUT_OF_MEMORY, "batch tables");

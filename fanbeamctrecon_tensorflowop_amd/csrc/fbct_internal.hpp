@@ -40,6 +40,19 @@ __host__ __device__ inline unsigned xcd_block(unsigned b, unsigned total)
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Logical block of workgroup b in a grid of `total`: consecutive chunks of kC logical
+// blocks each run on one XCD, the chunks dealt round-robin to the XCDs (blocks past the
+// last whole round keep their own id).  Fine-grained enough to balance work items of
+// unequal cost across the XCDs, coarse enough that one XCD's resident workgroups take
+// neighbouring items.
+__host__ __device__ inline unsigned xcd_chunk(unsigned b, unsigned total)
+{
+    constexpr unsigned kC = 64;
+    if (b >= total / (kNumXcd * kC) * (kNumXcd * kC)) return b;
+    const unsigned x = b % kNumXcd, k = b / kNumXcd;
+    return ((k / kC) * kNumXcd + x) * kC + k % kC;
+}
+
 // (tile, slice group) of workgroup b in a grid of `tiles` tiles x `groups` slice groups.
 // When the groups divide evenly over the XCDs, each XCD takes whole groups (their inputs
 // stay in its L2 and every XCD does the same work); otherwise the hardware order is kept.
@@ -51,21 +64,12 @@ __host__ __device__ inline void xcd_tile(unsigned b, unsigned tiles, unsigned gr
     tile = l % tiles;
 }
 
-// (tile x, tile y, slice group) of workgroup b in a grid of gx x gy tiles and `groups`
-// slice groups: each XCD takes a contiguous range of the logical order, which runs over
-// the slice groups, in each over blocks of kB x kB tiles (block rows top to bottom,
-// blocks left to right, partial blocks at the right and bottom edges) and in a block
-// over its tiles row by row.  The workgroups resident on one XCD at a time then cover a
-// compact square of the image, whose detector footprints in every view overlap, so the
-// columns they stage are shared through that XCD's L2.
-__host__ __device__ inline void xcd_blocked_tile(unsigned b, unsigned gx, unsigned gy, unsigned groups, unsigned &tx,
-                                                 unsigned &ty, unsigned &group)
+// Tile t of a gx x gy grid in blocked order: blocks of kB x kB tiles (block rows top to
+// bottom, blocks left to right, partial blocks at the right and bottom edges), in a block
+// its tiles row by row.
+__host__ __device__ inline void blocked_tile(unsigned t, unsigned gx, unsigned gy, unsigned &tx, unsigned &ty)
 {
     constexpr unsigned kB = 8;
-    const unsigned tiles = gx * gy;
-    const unsigned l = xcd_block(b, tiles * groups);
-    group = l / tiles;
-    const unsigned t = l % tiles;
     const unsigned br = t / (gx * kB);              // block row (full rows hold gx * kB tiles)
     const unsigned h = gy - br * kB < kB ? gy - br * kB : kB;  // its height
     const unsigned rem = t - br * gx * kB;
@@ -74,6 +78,21 @@ __host__ __device__ inline void xcd_blocked_tile(unsigned b, unsigned gx, unsign
     const unsigned r = rem - bc * kB * h;
     ty = br * kB + r / w;
     tx = bc * kB + r % w;
+}
+
+// (tile x, tile y, slice group) of workgroup b in a grid of gx x gy tiles and `groups`
+// slice groups: each XCD takes a contiguous range of the logical order, which runs over
+// the slice groups and in each over the tiles in blocked order.  The workgroups resident
+// on one XCD at a time then cover a compact square of the image, whose detector
+// footprints in every view overlap, so the columns they stage are shared through that
+// XCD's L2.
+__host__ __device__ inline void xcd_blocked_tile(unsigned b, unsigned gx, unsigned gy, unsigned groups, unsigned &tx,
+                                                 unsigned &ty, unsigned &group)
+{
+    const unsigned tiles = gx * gy;
+    const unsigned l = xcd_block(b, tiles * groups);
+    group = l / tiles;
+    blocked_tile(l % tiles, gx, gy, tx, ty);
 }
 
 // BP atan(z) = z + z^3 R(z^2): at most this many coefficients of R.

@@ -558,29 +558,33 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_kernel(FpaArgs A)
 // record.  Per view the candidate columns run in three ascending stretches: left of the
 // window (global memory), inside it (LDS only) and right of it (global memory), so the
 // hot middle loop carries no global addressing and no branch.
-template <int S>
+template <int S, int VB>
 struct CellStage {
-    static constexpr int kPitch = S + 4;  // floats per staged column: 16-byte multiple, off the bank period
-    float val[kStageVB][kStageCols][kPitch];
+    // floats per staged column: S + 4 (a 16-byte multiple, off the bank period) for the
+    // slice groups, one value for a single slice
+    static constexpr int kPitch = S >= 4 ? S + 4 : 1;
+    float val[VB][kStageCols][kPitch];
     // ray tables of the staged columns: (u0, w0, du, dw) and (1/du, 1/dw, N, du dw), each
     // a 16-byte record so that 16 lanes reading distinct columns hit distinct banks
-    float4 geo[kStageVB][kStageCols];
-    float4 aux[kStageVB][kStageCols];
+    float4 geo[VB][kStageCols];
+    float4 aux[VB][kStageCols];
 };
 
-template <int S>
+template <int S, int VB>
 union CellShared {
-    CellStage<S> stage;
-    float part[4 * kCombineS * kCells];
+    CellStage<S, VB> stage;
+    float part[4 * (S < kCombineS ? S : kCombineS) * kCells];
 };
 
-template <int S, typename TI, typename TO>
-__device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S> &sh, int tx, int ty, int s0)
+// VB views per staging batch (VB <= 16: lanes 4m + c evaluate the window corners).
+// S = 1 (the uneven batch's slices): wave 2 stages the values.
+template <int S, int VB, typename TI, typename TO>
+__device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S, VB> &sh, int tx, int ty, int s0)
 {
-    static_assert(S % 4 == 0, "a staged column is read four slices at a time");
-    constexpr int VB = kStageVB;
-    constexpr int SQ = S / 4;  // slices staged per wave
-    CellStage<S> &st = sh.stage;
+    static_assert(S == 1 || S % 4 == 0, "a staged column is read four slices at a time");
+    static_assert(VB >= 1 && VB <= 16, "window corners of at most 16 views per batch");
+    constexpr int SQ = S >= 4 ? S / 4 : 1;  // slices staged per wave
+    CellStage<S, VB> &st = sh.stage;
     int lx, ly;
     tile_cell(lx, ly);
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
@@ -647,6 +651,9 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             if (v0 + m >= A.nViews || c >= A.nCols) continue;
             const size_t rb = (size_t)(v0 + m) * A.nCols + c;
             const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+            if constexpr (S == 1) {
+                if (wave == 2) st.val[m][lane][0] = load_f_at(gbase, boff);
+            } else {
             float q4[SQ];
 #pragma unroll
             for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
@@ -660,6 +667,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
             } else {
 #pragma unroll
                 for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
+            }
             }
             if (wave == 0) {
                 st.geo[m][lane] = A.ray_geo[rb];
@@ -684,10 +692,14 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
                 const int lc = c - wlo;
                 const float4 ax = st.aux[m][lc];
                 float sv[S];
+                if constexpr (S == 1) {
+                    sv[0] = st.val[m][lc][0];
+                } else {
 #pragma unroll
-                for (int s = 0; s < S; s += 4) {
-                    const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
-                    sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
+                    for (int s = 0; s < S; s += 4) {
+                        const float4 t4 = *reinterpret_cast<const float4 *>(&st.val[m][lc][s]);
+                        sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
+                    }
                 }
                 cell_accumulate<S>(st.geo[m][lc], make_float2(ax.x, ax.y), ax.z, ax.w, sv, fci, fcj, acc);
             }
@@ -701,24 +713,30 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S>
 template <int S, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_cell_staged_kernel(FpaArgs A)
 {
-    __shared__ CellShared<S> sh;
+    __shared__ CellShared<S, kStageVB> sh;
     unsigned tx, ty, g;  // blocked, XCD-aware order (neighbouring tiles share staged columns in L2)
     xcd_blocked_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), gridDim.x, gridDim.y, gridDim.z,
                      tx, ty, g);
-    cell_staged_tile<S, TI, TO>(A, sh, (int)tx, (int)ty, (int)g * S);
+    cell_staged_tile<S, kStageVB, TI, TO>(A, sh, (int)tx, (int)ty, (int)g * S);
 }
 
 // Uneven batch: slice s has its own geometry (desc[s]); work item = (slice, tile), items
 // of slice s are [prefix[s], prefix[s+1]).  A slice whose samples stay in the cells
 // [-1, n-1] takes 15 x 15-pixel cell tiles, as a single-slice call does; any other slice
-// TX x (256/TX) gather tiles.  A fixed grid strides over all items.
-template <int TX, typename TI, typename TO>
+// TX x (256/TX) gather tiles.  A fixed grid strides over all items, the XCDs taking
+// them in chunks of 64, and a slice's tiles run in blocked order, so the tiles resident
+// on one XCD cover a compact part of one image and share their rays through its L2.  VB > 0: the cell tiles stage their sinogram columns and ray records
+// in LDS, VB views per batch (the same arithmetic in the same order as the unstaged
+// tile, VB = 0).
+template <int TX, int VB, typename TI, typename TO>
 __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const FpaSliceDesc *__restrict__ desc,
                                                                        const int *__restrict__ prefix, int n,
                                                                        int total, const TI *sino, TO *img)
 {
-    __shared__ float part[4 * kCells];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    __shared__ CellShared<1, (VB > 0 ? VB : 1)> sh;
+    float *part = sh.part;
+    const int first = (int)xcd_chunk(blockIdx.x, gridDim.x);
+    for (int item = first; item < total; item += gridDim.x) {
         int lo = 0, hi = n - 1;  // last slice with prefix[s] <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -749,13 +767,15 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
         A.c_abs = d.c_abs;
         A.far_x = d.far_x;
         A.far_y = d.far_y;
-        const int local = item - prefix[lo];
+        const unsigned local = (unsigned)(item - prefix[lo]);
+        unsigned tx, ty;
         if (adj_cells_fit(A.far_x, A.far_y)) {  // same test as the single-slice launch
-            const int ntx = adj_cell_tiles(A.nX);
-            cell_tile<1, TI, TO>(A, local % ntx, local / ntx, 0, part);
+            blocked_tile(local, adj_cell_tiles(A.nX), adj_cell_tiles(A.nY), tx, ty);
+            if constexpr (VB > 0) cell_staged_tile<1, VB, TI, TO>(A, sh, (int)tx, (int)ty, 0);
+            else cell_tile<1, TI, TO>(A, (int)tx, (int)ty, 0, part);
         } else {
-            const int ntx = (A.nX + TX - 1) / TX;
-            adjoint_tile<1, TX, TI, TO>(A, local % ntx, local / ntx, 0);
+            blocked_tile(local, (A.nX + TX - 1) / TX, (A.nY + 256 / TX - 1) / (256 / TX), tx, ty);
+            adjoint_tile<1, TX, TI, TO>(A, (int)tx, (int)ty, 0);
         }
     }
 }
@@ -837,21 +857,26 @@ fbct_status launch_fp_adjoint(const fbct_plan &p, const DeviceTables &t, const v
 }
 
 template <int TX, typename TI, typename TO>
-static void launch_fpa_batch_t(dim3 grid, hipStream_t stream, const BatchDevice &bd, const int *prefix, int n,
+static void launch_fpa_batch_t(int vb, dim3 grid, hipStream_t stream, const BatchDevice &bd, const int *prefix, int n,
                                int total, const void *sino, void *img)
 {
-    hipLaunchKernelGGL((fp_adjoint_batch_kernel<TX, TI, TO>), grid, dim3(kAdjThreads), 0, stream, bd.fpa_desc,
-                       prefix, n, total, static_cast<const TI *>(sino), static_cast<TO *>(img));
+    const TI *si = static_cast<const TI *>(sino);
+    TO *io = static_cast<TO *>(img);
+    switch (vb) {
+    case 4: hipLaunchKernelGGL((fp_adjoint_batch_kernel<TX, 4, TI, TO>), grid, dim3(kAdjThreads), 0, stream, bd.fpa_desc, prefix, n, total, si, io); break;
+    case 8: hipLaunchKernelGGL((fp_adjoint_batch_kernel<TX, 8, TI, TO>), grid, dim3(kAdjThreads), 0, stream, bd.fpa_desc, prefix, n, total, si, io); break;
+    default: hipLaunchKernelGGL((fp_adjoint_batch_kernel<TX, 0, TI, TO>), grid, dim3(kAdjThreads), 0, stream, bd.fpa_desc, prefix, n, total, si, io); break;
+    }
 }
 
 template <int TX>
-static void launch_fpa_batch_x(fbct_dtype ti, fbct_dtype to, dim3 grid, hipStream_t stream, const BatchDevice &bd,
-                               const int *prefix, int n, int total, const void *sino, void *img)
+static void launch_fpa_batch_x(fbct_dtype ti, fbct_dtype to, int vb, dim3 grid, hipStream_t stream,
+                               const BatchDevice &bd, const int *prefix, int n, int total, const void *sino, void *img)
 {
-    if (ti == FBCT_F32 && to == FBCT_F32) launch_fpa_batch_t<TX, float, float>(grid, stream, bd, prefix, n, total, sino, img);
-    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fpa_batch_t<TX, bf16_t, float>(grid, stream, bd, prefix, n, total, sino, img);
-    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fpa_batch_t<TX, float, bf16_t>(grid, stream, bd, prefix, n, total, sino, img);
-    else launch_fpa_batch_t<TX, bf16_t, bf16_t>(grid, stream, bd, prefix, n, total, sino, img);
+    if (ti == FBCT_F32 && to == FBCT_F32) launch_fpa_batch_t<TX, float, float>(vb, grid, stream, bd, prefix, n, total, sino, img);
+    else if (ti == FBCT_BF16 && to == FBCT_F32) launch_fpa_batch_t<TX, bf16_t, float>(vb, grid, stream, bd, prefix, n, total, sino, img);
+    else if (ti == FBCT_F32 && to == FBCT_BF16) launch_fpa_batch_t<TX, float, bf16_t>(vb, grid, stream, bd, prefix, n, total, sino, img);
+    else launch_fpa_batch_t<TX, bf16_t, bf16_t>(vb, grid, stream, bd, prefix, n, total, sino, img);
 }
 
 fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, const void *sino, fbct_dtype ti,
@@ -868,11 +893,12 @@ fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, 
                               : std::min<long long>(total, (long long)b.shape.adj_wgs_per_cu * bd.num_cu);
     const dim3 grid((unsigned)wgs);
     const int *prefix = bd.fpa_prefix[k];
+    const int vb = b.shape.adj_stage_views;
     switch (b.shape.adj_tile_x) {
-    case 8: launch_fpa_batch_x<8>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
-    case 32: launch_fpa_batch_x<32>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
-    case 64: launch_fpa_batch_x<64>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
-    default: launch_fpa_batch_x<16>(ti, to, grid, stream, bd, prefix, n, total, sino, img); break;
+    case 8: launch_fpa_batch_x<8>(ti, to, vb, grid, stream, bd, prefix, n, total, sino, img); break;
+    case 32: launch_fpa_batch_x<32>(ti, to, vb, grid, stream, bd, prefix, n, total, sino, img); break;
+    case 64: launch_fpa_batch_x<64>(ti, to, vb, grid, stream, bd, prefix, n, total, sino, img); break;
+    default: launch_fpa_batch_x<16>(ti, to, vb, grid, stream, bd, prefix, n, total, sino, img); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "fp_adjoint_batch_kernel launch: %s", hipGetErrorString(e));

@@ -282,6 +282,9 @@ typedef struct fbct_batch_shape {
                                other slices take fixed 15 x 15-pixel cell tiles      */
     int32_t adj_wgs_per_cu; /* FP^T workgroups per compute unit in the grid, 1..32;
                                0: one workgroup per work item                       */
+    int32_t adj_stage_views; /* FP^T cell tiles: views per LDS staging batch of the
+                               sinogram columns and ray records, 4 or 8; 0: no
+                               staging (rays read from global memory)               */
 } fbct_batch_shape;
 
 fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch **out);

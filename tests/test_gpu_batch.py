@@ -9,6 +9,7 @@ Checks, through the C ABI (fbct_batch_*):
   * BF16 storage equals the single-plan BF16 path bit for bit;
   * the HIP-graph-captured FP -> residual -> FP^T step equals the eager step on every
     replay, with eager work and new input data between replays;
+  * staging the cell tiles' rays in LDS (4 or 8 views per batch) changes no bit of FP^T;
   * the autotuner sets a valid shape and reads it back from its on-disk cache.
 """
 import numpy as np
@@ -91,6 +92,24 @@ def test_batch_shapes_agree(uneven, rays, tile, wgs):
     _say("shape rays=%d tile=%d wgs=%d: fp %.2e adj %.2e" % (rays, tile, wgs, ey, eg))
     assert ey <= 1e-6 and eg <= 1e-6
     batch.set_shape(fp_rays=64, fp_wgs_per_cu=8, adj_tile_x=16, adj_wgs_per_cu=8)
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("wgs", [3, 0])
+@pytest.mark.parametrize("views", [4, 8])
+def test_batch_staged_cells_bitwise(uneven, views, wgs, storage):
+    scans, phantoms, plans, batch = uneven
+    dt = torch.float32 if storage == "f32" else torch.bfloat16
+    x = batch.pack_images(phantoms)
+    batch.set_shape(fp_rays=64, fp_wgs_per_cu=8, adj_tile_x=16, adj_wgs_per_cu=8, adj_stage_views=0)
+    y = batch.fp(x).to(dt)
+    g0 = batch.fp_adjoint(y)
+    batch.set_shape(adj_wgs_per_cu=wgs, adj_stage_views=views)
+    g1 = batch.fp_adjoint(y)
+    torch.cuda.synchronize()
+    batch.set_shape(adj_wgs_per_cu=8, adj_stage_views=0)
+    _say("staged cells views=%d wgs=%d %s: max |diff| %.3e" % (views, wgs, storage, float((g1 - g0).abs().max())))
+    assert torch.equal(g1, g0)
 
 
 def test_batch_matches_oracle(uneven, oracle_mod):

@@ -155,10 +155,13 @@ def test_uneven_batch_host_side():
     assert list(np.diff(b.sino_offsets)) == [p.n_views * p.n_cols for p in plans]
     assert b.fp_samples == sum(p.fp_samples for p in plans)
     assert b.updates == sum(p.nx * p.ny * p.n_views for p in plans)
-    assert b.shape == {"fp_rays": 64, "fp_wgs_per_cu": 0, "adj_tile_x": 16, "adj_wgs_per_cu": 16}
-    b.set_shape(fp_rays=32, adj_tile_x=64, fp_wgs_per_cu=1, adj_wgs_per_cu=32)
-    assert b.shape == {"fp_rays": 32, "fp_wgs_per_cu": 1, "adj_tile_x": 64, "adj_wgs_per_cu": 32}
-    for bad in ({"fp_rays": 48}, {"adj_tile_x": 12}, {"fp_wgs_per_cu": -1}, {"adj_wgs_per_cu": 33}):
+    assert b.shape == {"fp_rays": 64, "fp_wgs_per_cu": 0, "adj_tile_x": 16, "adj_wgs_per_cu": 16,
+                       "adj_stage_views": 0}
+    b.set_shape(fp_rays=32, adj_tile_x=64, fp_wgs_per_cu=1, adj_wgs_per_cu=32, adj_stage_views=8)
+    assert b.shape == {"fp_rays": 32, "fp_wgs_per_cu": 1, "adj_tile_x": 64, "adj_wgs_per_cu": 32,
+                       "adj_stage_views": 8}
+    for bad in ({"fp_rays": 48}, {"adj_tile_x": 12}, {"fp_wgs_per_cu": -1}, {"adj_wgs_per_cu": 33},
+                {"adj_stage_views": 2}, {"adj_stage_views": 16}):
         with pytest.raises(_lib.FbctError) as e:
             b.set_shape(**bad)
         assert e.value.status == 1
