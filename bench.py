@@ -258,7 +258,8 @@ def _group_by_marker(disp, names):
         if cur is None:
             continue
         o = ops.setdefault(cur, {"kernels": [], "counters": {}, "ns": 0.0})
-        o["kernels"].append(e["kernel"].split("(")[0])
+        # the name without its argument list ("(anonymous namespace)" holds a parenthesis)
+        o["kernels"].append(e["kernel"].replace("(anonymous namespace)::", "").split("(")[0])
         o["ns"] += e["ns"]
         for k, v in e["counters"].items():
             o["counters"][k] = o["counters"].get(k, 0.0) + v
@@ -591,6 +592,11 @@ def _probe_rates(prof):
     return out
 
 
+def calibration_summary(cal):
+    """The calibration record as printed: scale factors rounded, probe rates as measured."""
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cal.items()}
+
+
 def _calibration(prof):
     """Scale factors and probe readings of a counter profile, or None.  GRBM_GUI_ACTIVE
     sums the 8 XCDs' busy cycles, so per-GPU-cycle rates divide by it / 8."""
@@ -662,7 +668,7 @@ def roofline(kernel, ms, op, prof, cal, flops=None, lds_bytes=None, model=""):
     # the kernels the profile saw for this operation (their launches, in order, once each)
     names = []
     for k in e["kernels"]:
-        k = k.replace("void ", "").replace("fbct::(anonymous namespace)::", "").replace("unsigned short", "bf16")
+        k = k.replace("void ", "").replace("fbct::", "").replace("unsigned short", "bf16")
         if k not in names:
             names.append(k)
     r["kernel"] = " + ".join(names) if names else kernel
@@ -691,7 +697,7 @@ def roofline(kernel, ms, op, prof, cal, flops=None, lds_bytes=None, model=""):
         "busy": {k: round(v, 4) for k, v in busy.items()},
         "l2_hit_rate": round(hit / (hit + miss), 4) if hit is not None and miss is not None and hit + miss > 0
         else None,
-        "lds_bank_conflict_per_active": round(c["SQ_LDS_BANK_CONFLICT"] / c["SQ_LDS_IDX_ACTIVE"], 4)
+        "lds_bank_conflict_per_active": round(c.get("SQ_LDS_BANK_CONFLICT", 0.0) / c["SQ_LDS_IDX_ACTIVE"], 4)
         if c.get("SQ_LDS_IDX_ACTIVE") else None,
         "valu_dual_issue_per_issue": round(c.get("SQ_ACTIVE_INST_VALU2", 0.0) / c["SQ_ACTIVE_INST_VALU"], 4)
         if c.get("SQ_ACTIVE_INST_VALU") else None,
@@ -724,6 +730,8 @@ def main(argv=None):
                          "(sides 256-768, 360-1200 views, 512-1024 bins) in one persistent launch per operator, "
                          "autotuned; fdk: cone-beam FDK of a 512x512x256 volume from 720 views of a 1024x256 "
                          "flat panel, BF16 projections")
+    ap.add_argument("--counters-out", default=None,
+                    help="also write the per-operation counter sums of the rocprofv3 passes (JSON) to this file")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="dispatch the calibration probes and each operator of the workload once (for rocprofv3 "
                          "counter passes), then exit")
@@ -737,6 +745,9 @@ def main(argv=None):
 
     # counter passes first: child processes, before this one touches the GPU
     prof = {"error": "disabled (--no-counters)"} if args.no_counters else collect_counters(args)
+    if args.counters_out:
+        with open(args.counters_out, "w") as f:
+            json.dump(prof, f, indent=1, sort_keys=True)
     cal = _calibration(prof) if "error" not in prof else None
 
     import torch
@@ -797,7 +808,7 @@ def main(argv=None):
             result = dict(out, n_gpus=1, steps=args.steps, warmup=args.warmup, scaling="weak", vs_baseline=None,
                           data="synthetic")
     if cal:
-        result["counter_calibration"] = {k: round(v, 4) for k, v in cal.items()}
+        result["counter_calibration"] = calibration_summary(cal)
     print(json.dumps(result), flush=True)
 
 

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "fbct_internal.hpp"
 
@@ -402,8 +403,10 @@ struct BpaHat {
 };
 
 // Calls fn(m, w) for the candidates m of line k (FOV range fr) of ray r, w the hat weight
-// (times 1/L^2 for the weighted form); n_minor bounds the candidate range.
-template <bool kWeighted, typename Fn>
+// (times 1/L^2 for the weighted form); n_minor bounds the candidate range.  kEdge: the
+// column may be the first or the last (clamped taps, cut at half a column); for every
+// other column the hat with its cut at |rel| = 1 is max(1 - |rel|, 0), the same value.
+template <bool kWeighted, bool kEdge = true, typename Fn>
 __device__ __forceinline__ void bpa_line(const BpaRay &r, const BpaHat &hh, float h, int k, int2 fr, int n_minor, Fn &&fn)
 {
     const double ms = fma(r.a, (double)k, r.b);
@@ -420,8 +423,13 @@ __device__ __forceinline__ void bpa_line(const BpaRay &r, const BpaHat &hh, floa
         const float L = fmaf(dl, r.pl, Lk);
         const float z = d * __builtin_amdgcn_rcpf(L);
         const float rel = z * fmaf(z * z, hh.c3, hh.icol);
-        float w = 1.0f - fabsf(fminf(fmaxf(rel, hh.clo), hh.chi));
-        w = (rel > hh.cut_lo && rel < hh.cut_hi) ? w : 0.0f;
+        float w;
+        if constexpr (kEdge) {
+            w = 1.0f - fabsf(fminf(fmaxf(rel, hh.clo), hh.chi));
+            w = (rel > hh.cut_lo && rel < hh.cut_hi) ? w : 0.0f;
+        } else {
+            w = fmaxf(1.0f - fabsf(rel), 0.0f);
+        }
         if constexpr (kWeighted) w *= __builtin_amdgcn_rcpf(fmaf(L, L, d * d));
         fn(m, w);
     }
@@ -457,6 +465,8 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int c = (int)cb * 64 + lane;
     const bool cvalid = c < A.nCols;
+    // the workgroup holds the first or the last column (uniform)
+    const bool edge_wg = cb == 0 || (int)cb * 64 + 63 >= A.nCols - 1;
     const int cc = min(c, A.nCols - 1);
     const size_t sl = (size_t)slice0 + (size_t)S * g;
     const float *img = A.img + sl * A.img_stride;
@@ -559,26 +569,32 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
         // lines outside its own range
         if (!wide) {
             if (nw == 0) continue;  // no candidates in this chunk (uniform)
+            auto staged = [&](auto edge) {
+                constexpr bool kEdge = decltype(edge)::value;
 #pragma unroll
-            for (int j = 0; j < kBpaVpt; ++j) {
-                if (own[j]) continue;
-                float lacc[S];
+                for (int j = 0; j < kBpaVpt; ++j) {
+                    if (own[j]) continue;
+                    float lacc[S];
 #pragma unroll
-                for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
-                for (int k = k0; k <= k1; ++k) {
-                    if (k < r[j].klo || k > r[j].khi) continue;
-                    // every candidate lies in the window (it is the union of the candidate
-                    // ranges, one pixel wider); the clamp only keeps reads inside the tile
-                    const TapT *row = &tile[(k - k0) * kBpaWin];
-                    bpa_line<kWeighted>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
-                        const TapT t = row[min(max(m - mlo, 0), nw - 1)];
+                    for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
+                    for (int k = k0; k <= k1; ++k) {
+                        if (k < r[j].klo || k > r[j].khi) continue;
+                        // every candidate lies in the window (it is the union of the
+                        // candidate ranges, one pixel wider); the clamp only keeps reads
+                        // inside the tile
+                        const TapT *row = &tile[(k - k0) * kBpaWin];
+                        bpa_line<kWeighted, kEdge>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
+                            const TapT t = row[min(max(m - mlo, 0), nw - 1)];
 #pragma unroll
-                        for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
-                    });
+                            for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                        });
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
                 }
-#pragma unroll
-                for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
-            }
+            };
+            if (edge_wg) staged(std::true_type{});
+            else staged(std::false_type{});
         } else {
 #pragma unroll
             for (int j = 0; j < kBpaVpt; ++j) {

@@ -690,7 +690,10 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
             for (; c <= min(chi, wlo - 1); ++c) global_ray(v, c);
             for (; c <= min(chi, whi); ++c) {
                 const int lc = c - wlo;
+                // the ray record is read before the values: LDS reads complete in order,
+                // so the sums can run while the values are still in flight
                 const float4 ax = st.aux[m][lc];
+                const float4 gg = st.geo[m][lc];
                 float sv[S];
                 if constexpr (S == 1) {
                     sv[0] = st.val[m][lc][0];
@@ -701,7 +704,8 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
                         sv[s] = t4.x, sv[s + 1] = t4.y, sv[s + 2] = t4.z, sv[s + 3] = t4.w;
                     }
                 }
-                cell_accumulate<S>(st.geo[m][lc], make_float2(ax.x, ax.y), ax.z, ax.w, sv, fci, fcj, acc);
+                __builtin_amdgcn_sched_barrier(0);  // keep all the reads ahead of the sums
+                cell_accumulate<S>(gg, make_float2(ax.x, ax.y), ax.z, ax.w, sv, fci, fcj, acc);
             }
             for (; c <= chi; ++c) global_ray(v, c);
         }
