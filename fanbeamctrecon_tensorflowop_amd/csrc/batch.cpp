@@ -218,7 +218,7 @@ fbct_status fbct_batch_create(fbct_plan *const *plans, int n_slices, fbct_batch 
                 if (!big[g]) order.push_back(g);
             b->fp_small_cap[k] = small_cap;
         }
-        b->shape = fbct_batch_shape{64, 0, 16, 16, 0};
+        b->shape = fbct_batch_shape{64, 0, 16, 0, 0};
     } catch (const std::bad_alloc &) {
         delete b;
         return fail(FBCT_ERR_OUT_OF_MEMORY, "batch tables");

@@ -633,51 +633,91 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
         }
     };
 
-    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
-        int c_lo[VB];
-        {
-            int wlo, whi;
-            candidate_rays(A, A.cs[min(v0 + (lane >> 2) % VB, A.nViews - 1)], cx, cy, kCellRho, wlo, whi);
+    // Staging is software-pipelined: while a batch is evaluated from LDS, the next batch's
+    // values and ray records are loaded into registers (each wave SQ slices of every
+    // view's columns, and the records of the views m = wave mod 4), and written to LDS
+    // between the two barriers that follow.
+    constexpr int kRecViews = (VB + 3) / 4;  // views whose records one wave loads
+    auto window = [&](int v0, int (&c_lo)[VB]) {
+        int wlo, whi;
+        candidate_rays(A, A.cs[min(v0 + (lane >> 2) % VB, A.nViews - 1)], cx, cy, kCellRho, wlo, whi);
 #pragma unroll
-            for (int sh2 = 1; sh2 <= 2; sh2 <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh2));
+        for (int sh2 = 1; sh2 <= 2; sh2 <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh2));
 #pragma unroll
-            for (int m = 0; m < VB; ++m)
-                c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kStageCols, 0));
-        }
-        __syncthreads();  // the previous batch's reads are done
+        for (int m = 0; m < VB; ++m)
+            c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kStageCols, 0));
+    };
+    float pv[VB][SQ];             // next batch: this wave's values of lane's column
+    float4 pg[kRecViews];         // its ray records: geo
+    float2 pi[kRecViews];         //   1 / steps
+    int pn[kRecViews];            //   samples
+    auto load_batch = [&](int v0, const int (&c_lo)[VB]) {
 #pragma unroll
         for (int m = 0; m < VB; ++m) {
-            const int c = c_lo[m] + lane;
-            if (v0 + m >= A.nViews || c >= A.nCols) continue;
-            const size_t rb = (size_t)(v0 + m) * A.nCols + c;
-            const unsigned boff = (unsigned)rb * (unsigned)sizeof(TI);
+            const int c = min(c_lo[m] + lane, A.nCols - 1);
+            const unsigned boff = (unsigned)((size_t)min(v0 + m, A.nViews - 1) * A.nCols + c) * (unsigned)sizeof(TI);
             if constexpr (S == 1) {
-                if (wave == 2) st.val[m][lane][0] = load_f_at(gbase, boff);
-            } else {
-            float q4[SQ];
-#pragma unroll
-            for (int q = 0; q < SQ; ++q) q4[q] = load_f_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
-            float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
-            if constexpr (SQ % 4 == 0) {
-#pragma unroll
-                for (int q = 0; q < SQ; q += 4)
-                    *reinterpret_cast<float4 *>(dst + q) = make_float4(q4[q], q4[q + 1], q4[q + 2], q4[q + 3]);
-            } else if constexpr (SQ == 2) {
-                *reinterpret_cast<float2 *>(dst) = make_float2(q4[0], q4[1]);
+                if (wave == 0) pv[m][0] = load_f_at(gbase, boff);
             } else {
 #pragma unroll
-                for (int q = 0; q < SQ; ++q) dst[q] = q4[q];
-            }
-            }
-            if (wave == 0) {
-                st.geo[m][lane] = A.ray_geo[rb];
-            } else if (wave == 1) {
-                const float2 iv = A.ray_inv[rb];
-                const float4 g = A.ray_geo[rb];
-                st.aux[m][lane] = make_float4(iv.x, iv.y, (float)A.ray_n[rb], g.z * g.w);
+                for (int q = 0; q < SQ; ++q)
+                    pv[m][q] = load_f_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
             }
         }
+#pragma unroll
+        for (int r = 0; r < kRecViews; ++r) {
+            const int m = wave + 4 * r;
+            if (m >= VB) break;
+            const int c = min(c_lo[m] + lane, A.nCols - 1);
+            const size_t rb = (size_t)min(v0 + m, A.nViews - 1) * A.nCols + c;
+            pg[r] = A.ray_geo[rb];
+            pi[r] = A.ray_inv[rb];
+            pn[r] = A.ray_n[rb];
+        }
+    };
+    auto store_batch = [&](int v0, const int (&c_lo)[VB]) {
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            if (v0 + m >= A.nViews || c_lo[m] + lane >= A.nCols) continue;
+            if constexpr (S == 1) {
+                if (wave == 0) st.val[m][lane][0] = pv[m][0];
+            } else {
+                float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
+                if constexpr (SQ % 4 == 0) {
+#pragma unroll
+                    for (int q = 0; q < SQ; q += 4)
+                        *reinterpret_cast<float4 *>(dst + q) =
+                            make_float4(pv[m][q], pv[m][q + 1], pv[m][q + 2], pv[m][q + 3]);
+                } else if constexpr (SQ == 2) {
+                    *reinterpret_cast<float2 *>(dst) = make_float2(pv[m][0], pv[m][1]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < SQ; ++q) dst[q] = pv[m][q];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRecViews; ++r) {
+            const int m = wave + 4 * r;
+            if (m >= VB || v0 + m >= A.nViews || c_lo[m] + lane >= A.nCols) continue;
+            st.geo[m][lane] = pg[r];
+            st.aux[m][lane] = make_float4(pi[r].x, pi[r].y, (float)pn[r], pg[r].z * pg[r].w);
+        }
+    };
+
+    int c_lo[VB], c_nx[VB];
+    window(0, c_nx);
+    load_batch(0, c_nx);
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
+#pragma unroll
+        for (int m = 0; m < VB; ++m) c_lo[m] = c_nx[m];
+        __syncthreads();  // the previous batch's reads are done
+        store_batch(v0, c_lo);
         __syncthreads();
+        if (v0 + VB < A.nViews) {
+            window(v0 + VB, c_nx);
+            load_batch(v0 + VB, c_nx);
+        }
         if (!active) continue;
 #pragma unroll 1
         for (int m = 0; m < VB; ++m) {

@@ -272,7 +272,8 @@ fbct_status fbct_fdk(fbct_plan *plan, const float *proj, float *vol, int n_slice
  * (fbct_batch_layout; offsets count elements of the storage type). */
 typedef struct fbct_batch fbct_batch;
 
-/* Launch shape of a batch's persistent kernels. */
+/* Launch shape of a batch's persistent kernels (a new batch: 64 rays, one FP
+ * workgroup per item, 16-pixel gather tiles, one FP^T workgroup per item, no staging). */
 typedef struct fbct_batch_shape {
     int32_t fp_rays;        /* rays per FP workgroup: 64 or 32                       */
     int32_t fp_wgs_per_cu;  /* FP workgroups per compute unit in the grid, 1..32;

@@ -155,7 +155,7 @@ def test_uneven_batch_host_side():
     assert list(np.diff(b.sino_offsets)) == [p.n_views * p.n_cols for p in plans]
     assert b.fp_samples == sum(p.fp_samples for p in plans)
     assert b.updates == sum(p.nx * p.ny * p.n_views for p in plans)
-    assert b.shape == {"fp_rays": 64, "fp_wgs_per_cu": 0, "adj_tile_x": 16, "adj_wgs_per_cu": 16,
+    assert b.shape == {"fp_rays": 64, "fp_wgs_per_cu": 0, "adj_tile_x": 16, "adj_wgs_per_cu": 0,
                        "adj_stage_views": 0}
     b.set_shape(fp_rays=32, adj_tile_x=64, fp_wgs_per_cu=1, adj_wgs_per_cu=32, adj_stage_views=8)
     assert b.shape == {"fp_rays": 32, "fp_wgs_per_cu": 1, "adj_tile_x": 64, "adj_wgs_per_cu": 32,
