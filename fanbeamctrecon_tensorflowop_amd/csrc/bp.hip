@@ -212,39 +212,57 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = 0.0f;
 
-    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
-        int c_lo[VB];
-        {
-            float xi, U, tt;
-            (void)column_of<kNc>(A, pc, min(v0 + (lane >> 2) % VB, A.nViews - 1), xi, U, tt);
-            int wlo = (int)floorf(fminf(fmaxf(xi - 0.5f, -1.0e6f), 1.0e6f));
+    // Staging is software-pipelined: while a batch is read from LDS, the next batch's
+    // columns are loaded into registers, and written to LDS between the two barriers that
+    // follow.
+    auto window = [&](int v0, int (&c_lo)[VB]) {
+        float xi, U, tt;
+        (void)column_of<kNc>(A, pc, min(v0 + (lane >> 2) % VB, A.nViews - 1), xi, U, tt);
+        int wlo = (int)floorf(fminf(fmaxf(xi - 0.5f, -1.0e6f), 1.0e6f));
 #pragma unroll
-            for (int sh = 1; sh <= 2; sh <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh));
+        for (int sh = 1; sh <= 2; sh <<= 1) wlo = min(wlo, __shfl_xor(wlo, sh));
 #pragma unroll
-            for (int m = 0; m < VB; ++m)
-                c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kBpStageCols, 0));
-        }
-        __syncthreads();  // the previous batch's reads are done
+        for (int m = 0; m < VB; ++m)
+            c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kBpStageCols, 0));
+    };
+    float q[VB][SQ];  // next batch: this wave's slices of lane's column, per view
+    auto load_batch = [&](int v0, const int (&c_lo)[VB]) {
 #pragma unroll
         for (int m = 0; m < VB; ++m) {
-            const int c = c_lo[m] + lane;
-            if (v0 + m >= A.nViews || c >= A.nCols) continue;
-            const float *src = sino_w + (size_t)(v0 + m) * A.nCols + c;
-            float q[SQ];
+            const float *src = sino_w + (size_t)min(v0 + m, A.nViews - 1) * A.nCols + min(c_lo[m] + lane, A.nCols - 1);
 #pragma unroll
-            for (int k = 0; k < SQ; ++k) q[k] = src[(size_t)k * A.sino_stride];
+            for (int k = 0; k < SQ; ++k) q[m][k] = src[(size_t)k * A.sino_stride];
+        }
+    };
+    auto store_batch = [&](int v0, const int (&c_lo)[VB]) {
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            if (v0 + m >= A.nViews || c_lo[m] + lane >= A.nCols) continue;
             float *dst = &st.val[m][lane][wave * SQ];
             if constexpr (SQ % 4 == 0) {
 #pragma unroll
                 for (int k = 0; k < SQ; k += 4)
-                    *reinterpret_cast<float4 *>(dst + k) = make_float4(q[k], q[k + 1], q[k + 2], q[k + 3]);
+                    *reinterpret_cast<float4 *>(dst + k) = make_float4(q[m][k], q[m][k + 1], q[m][k + 2], q[m][k + 3]);
             } else if constexpr (SQ == 2) {
-                *reinterpret_cast<float2 *>(dst) = make_float2(q[0], q[1]);
+                *reinterpret_cast<float2 *>(dst) = make_float2(q[m][0], q[m][1]);
             } else {
-                dst[0] = q[0];
+                dst[0] = q[m][0];
             }
         }
+    };
+    int c_lo[VB], c_nx[VB];
+    window(0, c_nx);
+    load_batch(0, c_nx);
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
+#pragma unroll
+        for (int m = 0; m < VB; ++m) c_lo[m] = c_nx[m];
+        __syncthreads();  // the previous batch's reads are done
+        store_batch(v0, c_lo);
         __syncthreads();
+        if (v0 + VB < A.nViews) {
+            window(v0 + VB, c_nx);
+            load_batch(v0 + VB, c_nx);
+        }
         if (!inside) continue;
 #pragma unroll 1
         for (int m = 0; m < VB; ++m) {
