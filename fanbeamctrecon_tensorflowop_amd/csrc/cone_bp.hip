@@ -156,7 +156,10 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_kernel(ConeArgs A)
 // detector, which reproduces clamp-to-edge), and every voxel reads its four taps from
 // LDS.  A column whose taps leave the staged window (a footprint taller than
 // lds_rows<KZ>() or wider than kLdsCols) reads them from global memory instead.
-constexpr int kLdsVB = 2;                      // views per staging batch
+// views per staging batch: with the next batch's rows loaded into registers while the
+// current one is evaluated, one view (126 VGPRs, four waves per SIMD) beat two (150, three
+// waves): 16.3 against 17.3 ms on the 512 x 512 x 256 FDK, 19.2 ms for two views unpipelined
+constexpr int kLdsVB = 1;
 constexpr int kLdsGroup = 16;                  // views per footprint evaluation (64 lanes / 4 corners)
 constexpr int kLdsCols = 64;                   // staged columns per row: one per lane
 constexpr int kLdsPitch = kLdsCols + 1;        // floats between the views of a staged row
@@ -166,9 +169,9 @@ constexpr int kLdsPitch = kLdsCols + 1;        // floats between the views of a 
 template <int VB>
 constexpr int lds_stride()
 {
-    // odd, and 10 beyond the packed VB * kLdsPitch: 141 for two views (strides 131 to 141
-    // measured: 141 has the fewest bank conflicts between the lanes' rows; up to 142 keeps
-    // four workgroups of 72 staged rows per CU)
+    // odd, and 10 beyond the packed VB * kLdsPitch: 75 for one view, 141 for two (strides
+    // 131 to 141 measured for two views: 141 had the fewest bank conflicts between the
+    // lanes' rows)
     return ((VB * kLdsPitch) | 1) + 10;
 }
 // Staged rows per view for z-chunks of KZ voxels: a 16 x 16 x KZ tile's taps span at most
@@ -238,122 +241,153 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
     const size_t view_elems = (size_t)nR * nC;
     const float *qvol = A.q + vi * (size_t)A.nViews * view_elems;
 
-    for (int vg = 0; vg < A.nViews; vg += kLdsGroup) {
-        // footprints of views vg + m, m < 16: after the 4-lane reduction lane 4m holds
-        // the first staged column, the first staged row and the last staged row (a margin
-        // of one sample on each side absorbs rounding between corner and voxel)
-        int f_c, f_r0, f_r1;
-        {
-            const float2 cs = A.cs[min(vg + (lane >> 2), A.nViews - 1)];
-            const float U = fmaf(-cy, cs.y, fmaf(-cx, cs.x, A.R));
-            const float tt = fmaf(cx, cs.y, -cy * cs.x);
-            const float rU = __builtin_amdgcn_rcpf(U);
-            const float zz = tt * rU;
-            float ctn = fmaf(kFlat ? zz : fan_atan<kNc>(A, zz), A.xi_scale, A.xi_offset) - 0.5f;
-            const float aa = rU * A.zr_scale;
-            const float ra = fmaf(z0, aa, A.zr_offset), rb = fmaf(z1, aa, A.zr_offset);
-            float rtn = fminf(ra, rb), rtx = fmaxf(ra, rb);
+    // footprints of views vg + m, m < 16: after the 4-lane reduction lane 4m holds the
+    // first staged column, the first staged row and the last staged row (a margin of one
+    // sample on each side absorbs rounding between corner and voxel)
+    auto footprints = [&](int vg, int &f_c, int &f_r0, int &f_r1) {
+        const float2 cs = A.cs[min(vg + (lane >> 2), A.nViews - 1)];
+        const float U = fmaf(-cy, cs.y, fmaf(-cx, cs.x, A.R));
+        const float tt = fmaf(cx, cs.y, -cy * cs.x);
+        const float rU = __builtin_amdgcn_rcpf(U);
+        const float zz = tt * rU;
+        float ctn = fmaf(kFlat ? zz : fan_atan<kNc>(A, zz), A.xi_scale, A.xi_offset) - 0.5f;
+        const float aa = rU * A.zr_scale;
+        const float ra = fmaf(z0, aa, A.zr_offset), rb = fmaf(z1, aa, A.zr_offset);
+        float rtn = fminf(ra, rb), rtx = fmaxf(ra, rb);
 #pragma unroll
-            for (int sh = 1; sh <= 2; sh <<= 1) {
-                ctn = fminf(ctn, __shfl_xor(ctn, sh));
-                rtn = fminf(rtn, __shfl_xor(rtn, sh));
-                rtx = fmaxf(rtx, __shfl_xor(rtx, sh));
-            }
-            f_c = floor_to_int(ctn) - 1;
-            f_r0 = floor_to_int(rtn) - 1;
-            f_r1 = floor_to_int(rtx) + 2;
+        for (int sh = 1; sh <= 2; sh <<= 1) {
+            ctn = fminf(ctn, __shfl_xor(ctn, sh));
+            rtn = fminf(rtn, __shfl_xor(rtn, sh));
+            rtx = fmaxf(rtx, __shfl_xor(rtx, sh));
         }
-        const int vend = min(vg + kLdsGroup, A.nViews);
-        for (int v0 = vg; v0 < vend; v0 += VB) {
-            int c_lo[VB], r_lo[VB], nrow[VB];
+        f_c = floor_to_int(ctn) - 1;
+        f_r0 = floor_to_int(rtn) - 1;
+        f_r1 = floor_to_int(rtx) + 2;
+    };
+    // window of the batch at v0 (inside the group at vg): first column, first row and
+    // row count per view (0 past the last view); wave-uniform
+    auto batch_info = [&](int v0, int vg, int f_c, int f_r0, int f_r1, int (&c_lo)[VB], int (&r_lo)[VB],
+                          int (&nrow)[VB]) {
 #pragma unroll
-            for (int m = 0; m < VB; ++m) {
-                const int src = 4 * (v0 - vg + m);
-                c_lo[m] = __builtin_amdgcn_readlane(f_c, src);
-                r_lo[m] = __builtin_amdgcn_readlane(f_r0, src);
-                const int r_hi = __builtin_amdgcn_readlane(f_r1, src);
-                nrow[m] = (v0 + m < vend) ? min(max(r_hi - r_lo[m] + 1, 1), kLdsRows) : 0;
+        for (int m = 0; m < VB; ++m) {
+            const int src = 4 * (v0 - vg + m);
+            c_lo[m] = __builtin_amdgcn_readlane(f_c, src);
+            r_lo[m] = __builtin_amdgcn_readlane(f_r0, src);
+            const int r_hi = __builtin_amdgcn_readlane(f_r1, src);
+            nrow[m] = (v0 + m < A.nViews) ? min(max(r_hi - r_lo[m] + 1, 1), kLdsRows) : 0;
+        }
+    };
+    // Staging is software-pipelined: while a batch is read from LDS, the next batch's
+    // rows (wave w: rows w, w + 4, ... of each view) are loaded into registers, and
+    // written to LDS between the two barriers that follow.
+    constexpr int kRw = kLdsRows / 4;  // staged rows per wave and view
+    static_assert(kLdsRows % 4 == 0, "rows are dealt to the 4 waves");
+    float pv[VB][kRw];
+    auto load_batch = [&](int v0, const int (&c_lo)[VB], const int (&r_lo)[VB], const int (&nrow)[VB]) {
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            if (nrow[m] == 0) continue;
+            const float *qv = qvol + (size_t)(v0 + m) * view_elems;
+            const unsigned c = (unsigned)min(max(c_lo[m] + lane, 0), nC - 1);
+#pragma unroll
+            for (int k = 0; k < kRw; ++k) {
+                const int rr = wave + 4 * k;
+                if (rr >= nrow[m]) break;
+                const int r = min(max(r_lo[m] + rr, 0), nR - 1);
+                pv[m][k] = (qv + (size_t)r * nC)[c];
             }
-            __syncthreads();  // the previous batch's reads are done
+        }
+    };
+    auto store_batch = [&](const int (&nrow)[VB]) {
 #pragma unroll
-            for (int m = 0; m < VB; ++m) {
-                if (nrow[m] == 0) continue;
-                const float *qv = qvol + (size_t)(v0 + m) * view_elems;
-                const unsigned c = (unsigned)min(max(c_lo[m] + lane, 0), nC - 1);
-                float *dst = stage + m * kLdsPitch + lane;
-                for (int i0 = wave; i0 < nrow[m]; i0 += 16) {
-                    float val[4];
+        for (int m = 0; m < VB; ++m) {
+            float *dst = stage + m * kLdsPitch + lane;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int r = min(max(r_lo[m] + min(i0 + 4 * u, nrow[m] - 1), 0), nR - 1);
-                        val[u] = (qv + (size_t)r * nC)[c];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) dst[min(i0 + 4 * u, nrow[m] - 1) * RS] = val[u];
-                }
+            for (int k = 0; k < kRw; ++k) {
+                const int rr = wave + 4 * k;
+                if (rr >= nrow[m]) break;
+                dst[rr * RS] = pv[m][k];
             }
-            __syncthreads();
-            if (!active) continue;
+        }
+    };
+
+    int f_c, f_r0, f_r1;
+    int c_lo[VB], r_lo[VB], nrow[VB], c_nx[VB], r_nx[VB], n_nx[VB];
+    footprints(0, f_c, f_r0, f_r1);
+    batch_info(0, 0, f_c, f_r0, f_r1, c_nx, r_nx, n_nx);
+    load_batch(0, c_nx, r_nx, n_nx);
+    for (int v0 = 0; v0 < A.nViews; v0 += VB) {
 #pragma unroll
-            for (int m = 0; m < VB; ++m) {
-                const int v = v0 + m;
-                if (v >= vend) break;
-                const float2 cs = A.cs[v];
-                const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
-                const float tv = fmaf(x, cs.y, -y * cs.x);
-                const float rU = __builtin_amdgcn_rcpf(U);
-                const float z = tv * rU;
-                const float xi = fmaf(kFlat ? z : fan_atan<kNc>(A, z), A.xi_scale, A.xi_offset);
-                bool in = xi > 0.0f && xi < (float)nC;
-                if (fabsf(xi) < kEdgeBand || fabsf(xi - (float)nC) < kEdgeBand) {
-                    const double2 c = A.csd[v];
-                    const double Ud = (double)A.R - xd * c.x - yd * c.y;
-                    const double td = xd * c.y - yd * c.x;
-                    in = (td - Ud * A.tan_min > 0.0) && (td - Ud * A.tan_max < 0.0);
+        for (int m = 0; m < VB; ++m) c_lo[m] = c_nx[m], r_lo[m] = r_nx[m], nrow[m] = n_nx[m];
+        __syncthreads();  // the previous batch's reads are done
+        store_batch(nrow);
+        __syncthreads();
+        const int v1 = v0 + VB;
+        if (v1 < A.nViews) {
+            if (v1 % kLdsGroup == 0) footprints(v1, f_c, f_r0, f_r1);
+            batch_info(v1, v1 - v1 % kLdsGroup, f_c, f_r0, f_r1, c_nx, r_nx, n_nx);
+            load_batch(v1, c_nx, r_nx, n_nx);
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int m = 0; m < VB; ++m) {
+            const int v = v0 + m;
+            if (v >= A.nViews) break;
+            const float2 cs = A.cs[v];
+            const float U = fmaf(-y, cs.y, fmaf(-x, cs.x, A.R));
+            const float tv = fmaf(x, cs.y, -y * cs.x);
+            const float rU = __builtin_amdgcn_rcpf(U);
+            const float z = tv * rU;
+            const float xi = fmaf(kFlat ? z : fan_atan<kNc>(A, z), A.xi_scale, A.xi_offset);
+            bool in = xi > 0.0f && xi < (float)nC;
+            if (fabsf(xi) < kEdgeBand || fabsf(xi - (float)nC) < kEdgeBand) {
+                const double2 c = A.csd[v];
+                const double Ud = (double)A.R - xd * c.x - yd * c.y;
+                const double td = xd * c.y - yd * c.x;
+                in = (td - Ud * A.tan_min > 0.0) && (td - Ud * A.tan_max < 0.0);
+            }
+            if (!in) continue;
+            const float w = kFlat ? rU * rU : __builtin_amdgcn_rcpf(fmaf(U, U, tv * tv));
+            const float a = rU * A.zr_scale;
+            const float step = a * A.dz;
+            const float rt0 = fmaf(z0, a, A.zr_offset);
+            const float rt1 = fmaf(step, (float)(KZ - 1), rt0);
+            const float ctu = xi - 0.5f;
+            const int cl = (int)floorf(ctu) - c_lo[m];
+            const int rla = (int)floorf(fminf(rt0, rt1)) - r_lo[m];
+            const int rlb = (int)floorf(fmaxf(rt0, rt1)) - r_lo[m];
+            if ((unsigned)cl <= (unsigned)(kLdsCols - 2) && rla >= 0 && rlb <= nrow[m] - 2) {
+                // unclamped coordinates: the staged rows and columns repeat the edges.
+                // Rows are blended first, then the columns with weights w (1 - al), w al.
+                const float al = __builtin_amdgcn_fractf(ctu);
+                const float wr = w * al, wl = w - wr;
+                const int base = m * kLdsPitch + cl - r_lo[m] * RS;
+#pragma unroll
+                for (int k = 0; k < KZ; ++k) {
+                    const float rt = fmaf(step, (float)k, rt0);
+                    const float be = __builtin_amdgcn_fractf(rt);
+                    const float *p = stage + (cvt_floor(rt) * RS + base);
+                    const float left = fmaf(be, p[RS] - p[0], p[0]);
+                    const float right = fmaf(be, p[RS + 1] - p[1], p[1]);
+                    acc[k] = fmaf(wl, left, fmaf(wr, right, acc[k]));
                 }
-                if (!in) continue;
-                const float w = kFlat ? rU * rU : __builtin_amdgcn_rcpf(fmaf(U, U, tv * tv));
-                const float a = rU * A.zr_scale;
-                const float step = a * A.dz;
-                const float rt0 = fmaf(z0, a, A.zr_offset);
-                const float rt1 = fmaf(step, (float)(KZ - 1), rt0);
-                const float ctu = xi - 0.5f;
-                const int cl = (int)floorf(ctu) - c_lo[m];
-                const int rla = (int)floorf(fminf(rt0, rt1)) - r_lo[m];
-                const int rlb = (int)floorf(fmaxf(rt0, rt1)) - r_lo[m];
-                if ((unsigned)cl <= (unsigned)(kLdsCols - 2) && rla >= 0 && rlb <= nrow[m] - 2) {
-                    // unclamped coordinates: the staged rows and columns repeat the edges.
-                    // Rows are blended first, then the columns with weights w (1 - al), w al.
-                    const float al = __builtin_amdgcn_fractf(ctu);
-                    const float wr = w * al, wl = w - wr;
-                    const int base = m * kLdsPitch + cl - r_lo[m] * RS;
+            } else {
+                const float *qv = qvol + (size_t)v * view_elems;
+                const float ct = fminf(fmaxf(ctu, 0.0f), cmax);
+                const float fc = fminf(floorf(ct), cfl);
+                const float al = ct - fc;
+                const int c0 = (int)fc, c1 = min(c0 + 1, nC - 1);
 #pragma unroll
-                    for (int k = 0; k < KZ; ++k) {
-                        const float rt = fmaf(step, (float)k, rt0);
-                        const float be = __builtin_amdgcn_fractf(rt);
-                        const float *p = stage + (cvt_floor(rt) * RS + base);
-                        const float left = fmaf(be, p[RS] - p[0], p[0]);
-                        const float right = fmaf(be, p[RS + 1] - p[1], p[1]);
-                        acc[k] = fmaf(wl, left, fmaf(wr, right, acc[k]));
-                    }
-                } else {
-                    const float *qv = qvol + (size_t)v * view_elems;
-                    const float ct = fminf(fmaxf(ctu, 0.0f), cmax);
-                    const float fc = fminf(floorf(ct), cfl);
-                    const float al = ct - fc;
-                    const int c0 = (int)fc, c1 = min(c0 + 1, nC - 1);
-#pragma unroll
-                    for (int k = 0; k < KZ; ++k) {
-                        const float rt = fminf(fmaxf(fmaf(step, (float)k, rt0), 0.0f), rmax);
-                        const float fr = fminf(floorf(rt), rfl);
-                        const float be = rt - fr;
-                        const int r0 = (int)fr, r1 = min(r0 + 1, nR - 1);
-                        const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
-                        const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
-                        const float top = fmaf(al, p01 - p00, p00);
-                        const float bot = fmaf(al, p11 - p10, p10);
-                        acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
-                    }
+                for (int k = 0; k < KZ; ++k) {
+                    const float rt = fminf(fmaxf(fmaf(step, (float)k, rt0), 0.0f), rmax);
+                    const float fr = fminf(floorf(rt), rfl);
+                    const float be = rt - fr;
+                    const int r0 = (int)fr, r1 = min(r0 + 1, nR - 1);
+                    const float p00 = qv[r0 * nC + c0], p01 = qv[r0 * nC + c1];
+                    const float p10 = qv[r1 * nC + c0], p11 = qv[r1 * nC + c1];
+                    const float top = fmaf(al, p01 - p00, p00);
+                    const float bot = fmaf(al, p11 - p10, p10);
+                    acc[k] = fmaf(w, fmaf(be, bot - top, top), acc[k]);
                 }
             }
         }

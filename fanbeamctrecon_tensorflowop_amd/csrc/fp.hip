@@ -141,12 +141,11 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
     }
     const bool asc = da >= 0.0f;
     // cell walk constants: reciprocal steps (a zero step never crosses: a large finite
-    // value), the boundary offset of a cell in the walking direction, da * dm
+    // value), the boundary offset of a cell in the walking direction
     const float ia = da != 0.0f ? 1.0f / da : 1.0e30f;
     const float im = dm != 0.0f ? 1.0f / dm : 1.0e30f;
     const float oa = da >= 0.0f ? 1.0f : 0.0f;
     const float om = dm >= 0.0f ? 1.0f : 0.0f;
-    const float dadm = da * dm;
     const TI *img[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -245,6 +244,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
         // in either cell: the bilinear interpolant is continuous there.
         const int kl = asc ? kstart : kown;
         const float kend = (float)(asc ? kown : kstart);
+        const float kend1 = kend - 1.0f;
         float acc[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] = 0.0f;
@@ -253,15 +253,18 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
             const float m = fmaf(kf, dm, m0);
             const float fa = floorf(a), fm = floorf(m);
             const float ta = a - fa, tm = m - fm;
-            // samples before the next boundary along each axis: floor(distance / step) + 1
-            const float ja = floorf((fa + oa - a) * ia);
-            const float jm = floorf((fm + om - m) * im);
-            const float n = fminf(fminf(ja, jm) + 1.0f, kend - kf);
+            // samples before the next boundary along each axis: floor(distance / step) + 1;
+            // the distance (fa + oa) - a is formed as oa - ta (ta is exact, so both are the
+            // one rounding of the same difference), and the three counts share one min
+            const float ja = floorf((oa - ta) * ia);
+            const float jm = floorf((om - tm) * im);
+            const float n = __builtin_amdgcn_fmed3f(fminf(ja, jm), -1.0e30f, kend1 - kf) + 1.0f;
             const float t1 = 0.5f * fmaf(n, n, -n);                        // sum of j
             const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);      // sum of j^2
             const float sa = fmaf(da, t1, n * ta);                          // sum of a-fractions
             const float sm = fmaf(dm, t1, n * tm);                          // sum of m-fractions
-            const float sam = fmaf(n, ta * tm, fmaf(fmaf(ta, dm, tm * da), t1, dadm * t2));
+            // sum of products: sum_j (ta + j da)(tm + j dm) = ta sm + da (tm t1 + dm t2)
+            const float sam = fmaf(ta, sm, da * fmaf(tm, t1, dm * t2));
             const float w11 = sam, w01 = sa - sam, w10 = sm - sam, w00 = (n - sa) - w10;
             const unsigned o = (unsigned)fmaf(fm, kRowB, fmaf(fa, kColB, obase));
             const TapT t00 = *reinterpret_cast<LdsTap *>((size_t)o);
