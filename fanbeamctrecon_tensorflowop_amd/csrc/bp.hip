@@ -538,6 +538,17 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
         for (int s = 0; s < S; ++s) tp[s] = img[off + (size_t)s * A.img_stride];
         return t;
     };
+    // the rare global-memory paths (wide windows, rays on their own axis): one slice at a
+    // time, so that they hold few registers (the scheduling barrier keeps the loads from
+    // being hoisted together); the same FMAs in the same order as the staged path
+    auto add_taps = [&](size_t off, float w, float (&lacc)[S]) {
+        const float *p = img + off;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            lacc[s] = fmaf(w, p[(size_t)s * A.img_stride], lacc[s]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
     double accd[kBpaVpt][S];
 #pragma unroll
     for (int j = 0; j < kBpaVpt; ++j)
@@ -622,9 +633,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
                 for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
                 for (int k = max(k0, r[j].klo); k <= min(k1, r[j].khi); ++k)
                     bpa_line<kWeighted>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
-                        const TapT t = load_taps((size_t)k * s_major + (size_t)m * s_minor);
-#pragma unroll
-                        for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                        add_taps((size_t)k * s_major + (size_t)m * s_minor, w, lacc);
                     });
 #pragma unroll
                 for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
@@ -651,9 +660,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
             for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
             for (int k = k0; k <= min(k0 + kBpaChunk - 1, r[j].khi); ++k)
                 bpa_line<kWeighted>(r[j], hh, hj, k, fv[k], nm, [&](int m, float w) {
-                    const TapT t = load_taps((size_t)k * sj + (size_t)m * si);
-#pragma unroll
-                    for (int s = 0; s < S; ++s) lacc[s] = fmaf(w, tap_at<S>(t, s), lacc[s]);
+                    add_taps((size_t)k * sj + (size_t)m * si, w, lacc);
                 });
 #pragma unroll
             for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];

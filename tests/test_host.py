@@ -304,3 +304,27 @@ def test_parker_weights_match_oracle(oracle_mod):
     with pytest.raises(_lib.FbctError) as e:
         short.parker_weights()
     assert e.value.status == 2
+
+
+def test_candidate_atan_polynomial_bound():
+    """The FP^T candidate ranges take the fan angle from a degree-7 odd polynomial on
+    |q| <= 1/2 and widen the range by kCandAtanErr; check, with the coefficients read from
+    the kernel source and evaluated in FP32 as the kernel does (Horner in q^2), that the
+    polynomial stays within that bound of atan (an underestimated bound could drop a
+    candidate ray and change the adjoint)."""
+    import re
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "fanbeamctrecon_tensorflowop_amd", "csrc", "fp_adjoint.hip")).read()
+    body = src[src.index("float cand_atan(float q)"):]
+    body = body[:body.index("}")]
+    coef = [float(v) for v in re.findall(r"([-+]?\d\.\d+e[-+]\d+)f", body)]
+    assert len(coef) == 4  # a3, a2, a1, a0 in the order of the Horner nest
+    bound = float(re.search(r"kCandAtanErr = ([0-9.e+-]+)f", src).group(1))
+    q = np.linspace(-0.5, 0.5, 400001).astype(np.float32)
+    s = (q * q).astype(np.float32)
+    r = np.float32(coef[0])
+    for c in coef[1:]:
+        r = (r * s + np.float32(c)).astype(np.float32)
+    y = (q * r).astype(np.float32)
+    err = np.abs(y.astype(np.float64) - np.arctan(q.astype(np.float64))).max()
+    assert err < 0.5 * bound, err
