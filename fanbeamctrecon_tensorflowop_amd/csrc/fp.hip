@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <climits>
 #include <mutex>
+#include <type_traits>
 
 #include "dtype.hpp"
 #include "fbct_internal.hpp"
@@ -141,11 +142,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
     }
     const bool asc = da >= 0.0f;
     // cell walk constants: reciprocal steps (a zero step never crosses: a large finite
-    // value), the boundary offset of a cell in the walking direction
+    // value) and, with the boundary offset o of a cell in the walking direction (1 or 0),
+    // the terms of (o - t) / step + 1 = t (-1/step) + (o / step + 1) for a fraction t
     const float ia = da != 0.0f ? 1.0f / da : 1.0e30f;
     const float im = dm != 0.0f ? 1.0f / dm : 1.0e30f;
-    const float oa = da >= 0.0f ? 1.0f : 0.0f;
-    const float om = dm >= 0.0f ? 1.0f : 0.0f;
+    const float nia = -ia, nim = -im;
+    const float ca = da >= 0.0f ? ia + 1.0f : 1.0f;
+    const float cm = dm >= 0.0f ? im + 1.0f : 1.0f;
     const TI *img[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -212,26 +215,35 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
             for (int s = 0; s < S; ++s) vp[s] = load_f_at(img[s], go * (unsigned)sizeof(TI));
             reinterpret_cast<TapT *>(tile)[mm * kPitch + aa] = val;
         };
-        if (xmajor) {  // a = i: kRowsPerPass rows of kTaps taps per pass of the workgroup
-            constexpr int kRowsPerPass = kFpThreads / kTaps;
-            if (tid < kRowsPerPass * kTaps) {
-                const int aa = tid % kTaps;
-                const unsigned ga = (unsigned)min(max(alo + aa, 0), na - 1);
+        // A tile inside the image (most of them) needs no clamping: its addresses then
+        // advance by a constant per pass (uniform choice per slab).
+        auto stage_tile = [&](auto clamp) {
+            constexpr bool kClamp = decltype(clamp)::value;
+            auto ca = [&](int a) { return kClamp ? min(max(a, 0), na - 1) : a; };
+            auto cm = [&](int m) { return kClamp ? min(max(m, 0), nm - 1) : m; };
+            if (xmajor) {  // a = i: kRowsPerPass rows of kTaps taps per pass of the workgroup
+                constexpr int kRowsPerPass = kFpThreads / kTaps;
+                if (tid < kRowsPerPass * kTaps) {
+                    const int aa = tid % kTaps;
+                    const unsigned ga = (unsigned)ca(alo + aa);
 #pragma unroll 2  // two taps' loads in flight per thread
-                for (int mm = tid / kTaps; mm < M; mm += kRowsPerPass)
-                    stage(aa, mm, (unsigned)min(max(mlo + mm, 0), nm - 1) * (unsigned)A.nX + ga);
-            }
-        } else {  // a = j: 64 minor taps per wave, one major row per wave and pass
-            const int wave = tid >> 6;
-            for (int m0 = 0; m0 < M; m0 += 64) {
-                const int mm = m0 + (tid & 63);
-                if (mm >= M) continue;
-                const unsigned gm = (unsigned)min(max(mlo + mm, 0), nm - 1);
+                    for (int mm = tid / kTaps; mm < M; mm += kRowsPerPass)
+                        stage(aa, mm, (unsigned)cm(mlo + mm) * (unsigned)A.nX + ga);
+                }
+            } else {  // a = j: 64 minor taps per wave, one major row per wave and pass
+                const int wave = tid >> 6;
+                for (int m0 = 0; m0 < M; m0 += 64) {
+                    const int mm = m0 + (tid & 63);
+                    if (mm >= M) continue;
+                    const unsigned gm = (unsigned)cm(mlo + mm);
 #pragma unroll 2
-                for (int aa = wave; aa < kTaps; aa += kFpThreads / 64)
-                    stage(aa, mm, (unsigned)min(max(alo + aa, 0), na - 1) * (unsigned)A.nX + gm);
+                    for (int aa = wave; aa < kTaps; aa += kFpThreads / 64)
+                        stage(aa, mm, (unsigned)ca(alo + aa) * (unsigned)A.nX + gm);
+                }
             }
-        }
+        };
+        if (alo >= 0 && alo + kSlab <= na - 1 && mlo >= 0 && mhi <= nm - 1) stage_tile(std::false_type{});
+        else stage_tile(std::true_type{});
         __syncthreads();  // tile ready
 
         // byte offset of tap (fa, fm) = (fm - mlo) * kRowB + (fa - alo) * kColB, exact in FP32
@@ -244,7 +256,6 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
         // in either cell: the bilinear interpolant is continuous there.
         const int kl = asc ? kstart : kown;
         const float kend = (float)(asc ? kown : kstart);
-        const float kend1 = kend - 1.0f;
         float acc[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[s] = 0.0f;
@@ -253,12 +264,13 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
             const float m = fmaf(kf, dm, m0);
             const float fa = floorf(a), fm = floorf(m);
             const float ta = a - fa, tm = m - fm;
-            // samples before the next boundary along each axis: floor(distance / step) + 1;
-            // the distance (fa + oa) - a is formed as oa - ta (ta is exact, so both are the
-            // one rounding of the same difference), and the three counts share one min
-            const float ja = floorf((oa - ta) * ia);
-            const float jm = floorf((om - tm) * im);
-            const float n = __builtin_amdgcn_fmed3f(fminf(ja, jm), -1.0e30f, kend1 - kf) + 1.0f;
+            // samples before the next boundary along each axis: floor(distance / step) + 1
+            // with distance (fa + o) - a = o - ta, one FMA each; the three counts share one
+            // min.  A count that rounding puts one past a boundary only moves a sample lying
+            // on it (within rounding) into this cell, where its bilinear weights are the same.
+            const float ja1 = floorf(fmaf(ta, nia, ca));
+            const float jm1 = floorf(fmaf(tm, nim, cm));
+            const float n = __builtin_amdgcn_fmed3f(fminf(ja1, jm1), -1.0e30f, kend - kf);
             const float t1 = 0.5f * fmaf(n, n, -n);                        // sum of j
             const float t2 = t1 * fmaf(n, 2.0f / 3.0f, -1.0f / 3.0f);      // sum of j^2
             const float sa = fmaf(da, t1, n * ta);                          // sum of a-fractions
