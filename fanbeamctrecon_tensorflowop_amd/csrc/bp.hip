@@ -27,6 +27,7 @@
 #include <climits>
 #include <type_traits>
 
+#include "dtype.hpp"
 #include "fbct_internal.hpp"
 
 namespace fbct {
@@ -531,21 +532,25 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
     const int n_minor = along_y ? A.nX : A.nY;
     const int2 *fov = along_y ? A.fov_row : A.fov_col;
     const float h = along_y ? (float)A.dx : (float)A.dy;
-    auto load_taps = [&](size_t off) {
+    // pixel off of the group's S slices: 32-bit byte offsets from the group's first image
+    // (the launch keeps S images under 4 GiB)
+    const unsigned sbytes = (unsigned)A.img_stride * (unsigned)sizeof(float);
+    auto load_taps = [&](unsigned off) {
         TapT t;
         float *tp = reinterpret_cast<float *>(&t);
+        const unsigned b = off * (unsigned)sizeof(float);
 #pragma unroll
-        for (int s = 0; s < S; ++s) tp[s] = img[off + (size_t)s * A.img_stride];
+        for (int s = 0; s < S; ++s) tp[s] = load_f_at(img, b + (unsigned)s * sbytes);
         return t;
     };
     // the rare global-memory paths (wide windows, rays on their own axis): one slice at a
     // time, so that they hold few registers (the scheduling barrier keeps the loads from
     // being hoisted together); the same FMAs in the same order as the staged path
-    auto add_taps = [&](size_t off, float w, float (&lacc)[S]) {
-        const float *p = img + off;
+    auto add_taps = [&](unsigned off, float w, float (&lacc)[S]) {
+        const unsigned b = off * (unsigned)sizeof(float);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            lacc[s] = fmaf(w, p[(size_t)s * A.img_stride], lacc[s]);
+            lacc[s] = fmaf(w, load_f_at(img, b + (unsigned)s * sbytes), lacc[s]);
             __builtin_amdgcn_sched_barrier(0);
         }
     };
@@ -584,13 +589,18 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
         const int nl = k1 - k0 + 1;
         if (threadIdx.x < (unsigned)nl) fovc[threadIdx.x] = fov[k0 + threadIdx.x];
         if (nw > 0) {
-            // rows: consecutive threads along the row; columns: along the chunk's lines,
-            // which are consecutive pixels of an image row
-            for (int e = (int)threadIdx.x; e < nl * nw; e += 256) {
-                int li, mm;
-                if (along_y) li = e / nw, mm = e - li * nw;
-                else mm = e / nl, li = e - mm * nl;
-                tile[li * kBpaWin + mm] = load_taps((size_t)(k0 + li) * s_major + (size_t)(mlo + mm) * s_minor);
+            // rows (lines along x): a wave per line, lanes along it; columns: 8 lanes along
+            // the chunk's lines, which are consecutive pixels of an image row
+            static_assert(kBpaChunk == 8, "column staging puts the chunk's lines on 8 lanes");
+            if (along_y) {
+                for (int li = wave; li < nl; li += 4)
+                    for (int mm = lane; mm < nw; mm += 64)
+                        tile[li * kBpaWin + mm] = load_taps((unsigned)((k0 + li) * A.nX + mlo + mm));
+            } else {
+                const int li = lane & 7;
+                if (li < nl)
+                    for (int mm = (int)threadIdx.x >> 3; mm < nw; mm += 32)
+                        tile[li * kBpaWin + mm] = load_taps((unsigned)((mlo + mm) * A.nX + k0 + li));
             }
         }
         __syncthreads();  // tile ready
@@ -633,7 +643,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
                 for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
                 for (int k = max(k0, r[j].klo); k <= min(k1, r[j].khi); ++k)
                     bpa_line<kWeighted>(r[j], hh, h, k, fovc[k - k0], n_minor, [&](int m, float w) {
-                        add_taps((size_t)k * s_major + (size_t)m * s_minor, w, lacc);
+                        add_taps((unsigned)(k * s_major + m * s_minor), w, lacc);
                     });
 #pragma unroll
                 for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
@@ -660,7 +670,7 @@ __global__ __launch_bounds__(256) void bp_adjoint_staged_kernel(BpaArgs A, int s
             for (int s = 0; s < S; ++s) lacc[s] = 0.0f;
             for (int k = k0; k <= min(k0 + kBpaChunk - 1, r[j].khi); ++k)
                 bpa_line<kWeighted>(r[j], hh, hj, k, fv[k], nm, [&](int m, float w) {
-                    add_taps((size_t)k * sj + (size_t)m * si, w, lacc);
+                    add_taps((unsigned)(k * sj + m * si), w, lacc);
                 });
 #pragma unroll
             for (int s = 0; s < S; ++s) accd[j][s] += (double)lacc[s];
@@ -778,9 +788,14 @@ fbct_status launch_bp_adjoint(const fbct_plan &p, const DeviceTables &t, const f
     // the same workgroups (kBpaV views per thread), so a slice's result does not depend
     // on the group it falls in
     static_assert(kBpaS == 8, "group sizes 8, 4, 2, 1");
+    // a group's images are addressed by 32-bit byte offsets: its S images stay below 4 GiB
+    const unsigned long long img_bytes = (unsigned long long)p.g.nX * p.g.nY * sizeof(float);
+    if (img_bytes >= (1ULL << 32))
+        return fail(FBCT_ERR_UNSUPPORTED, "fbct_bp_adjoint: an image of %llu bytes exceeds 4 GiB", img_bytes);
     const dim3 g2((p.g.nCols + 63) / 64, (p.g.nViews + 4 * kBpaV - 1) / (4 * kBpaV), 1);
     int done = 0;
     for (int S : {8, 4, 2, 1}) {
+        if ((unsigned long long)S * img_bytes >= (1ULL << 32)) continue;
         const int groups = (n - done) / S;
         if (groups == 0) continue;
         const dim3 grid(g2.x, g2.y, groups);

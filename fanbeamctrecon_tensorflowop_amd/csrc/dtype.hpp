@@ -33,6 +33,26 @@ __device__ __forceinline__ float load_f_at(const bf16_t *p, unsigned byte_off)
     return __uint_as_float(static_cast<unsigned>(b) << 16);
 }
 
+// *(base + byte_off): a 32-bit offset from the base (a scalar base takes it as the lane
+// offset of the load)
+template <typename T>
+__device__ __forceinline__ T load_at(const T *base, unsigned byte_off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// p, known to be the same in every lane and to point to global memory: a scalar base
+// the loads through it can take as their address (32-bit lane offsets, global loads).
+template <typename T>
+__device__ __forceinline__ const T *uniform_global(const T *p)
+{
+    const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const auto *g = reinterpret_cast<const __attribute__((address_space(1))) T *>(((unsigned long long)hi << 32) | lo);
+    return (const T *)g;
+}
+
 __device__ __forceinline__ bf16_t f32_to_bf16(float v)
 {
     const unsigned u = __float_as_uint(v);

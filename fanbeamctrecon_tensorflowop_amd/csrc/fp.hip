@@ -345,11 +345,13 @@ __global__ __launch_bounds__(kFpThreads) void fp_batch_kernel(const FpSliceDesc 
         A.target = nullptr;
         A.img = img + d.img_off;
         A.sino = sino + d.sino_off;
-        A.ray_geo = d.ray_geo;
-        A.ray_n = d.ray_n;
-        A.major = d.major[cfg];
-        A.slabs = d.slabs[cfg];
-        A.rows = d.rows[cfg];
+        // the slice's tables as workgroup-uniform global-memory pointers (read from the
+        // descriptor they would be generic, per-lane addresses: flat loads)
+        A.ray_geo = uniform_global(d.ray_geo);
+        A.ray_n = uniform_global(d.ray_n);
+        A.major = uniform_global(d.major[cfg]);
+        A.slabs = uniform_global(d.slabs[cfg]);
+        A.rows = uniform_global(d.rows[cfg]);
         A.img_stride = 0;
         A.sino_stride = 0;
         A.nX = d.nX;

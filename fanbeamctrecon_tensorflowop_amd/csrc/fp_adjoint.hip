@@ -544,12 +544,15 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
         for (int v = 0; v < A.nViews; ++v) {
             int clo, chi;
             candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
-            const size_t rbase = (size_t)v * A.nCols;
+            // 32-bit element offsets from the tables' bases (a plan's tables hold fewer than
+            // 2^26 rays)
+            const unsigned rbase = (unsigned)v * (unsigned)A.nCols;
             for (int c = clo; c <= chi; ++c) {
-                const float N = (float)A.ray_n[rbase + c];
-                const float4 g = A.ray_geo[rbase + c];
-                const float2 inv = A.ray_inv[rbase + c];
-                const unsigned boff = (unsigned)(rbase + c) * (unsigned)sizeof(TI);
+                const unsigned rb = rbase + (unsigned)c;
+                const float N = (float)load_at(A.ray_n, rb * 4u);
+                const float4 g = load_at(A.ray_geo, rb * 16u);
+                const float2 inv = load_at(A.ray_inv, rb * 8u);
+                const unsigned boff = rb * (unsigned)sizeof(TI);
                 float sv[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
@@ -806,10 +809,12 @@ __global__ __launch_bounds__(kAdjThreads) void fp_adjoint_batch_kernel(const Fpa
         FpaArgs A;
         A.sino = sino + d.sino_off;
         A.img = img + d.img_off;
-        A.ray_geo = d.ray_geo;
-        A.ray_n = d.ray_n;
-        A.ray_inv = d.ray_inv;
-        A.cs = d.cs;
+        // the slice's tables, as workgroup-uniform global-memory pointers (read from the
+        // descriptor they would be generic, per-lane addresses: flat loads)
+        A.ray_geo = uniform_global(d.ray_geo);
+        A.ray_n = uniform_global(d.ray_n);
+        A.ray_inv = uniform_global(d.ray_inv);
+        A.cs = uniform_global(d.cs);
         A.img_stride = 0;
         A.sino_stride = 0;
         A.nX = d.nX;
