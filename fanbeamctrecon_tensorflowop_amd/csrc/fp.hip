@@ -74,25 +74,19 @@ __device__ __forceinline__ float lane_of(const typename Taps<S>::T &t, int s)
 
 // Smallest k in [0, N] with P(k) true, where P(k) = (a0 + k da >= t) for da >= 0 and
 // (a0 + k da < t) for da < 0, evaluated with the same fmaf the samples use (monotone
-// in k, so P is false...false true...true).  ia = 1/da estimates k; the two scans make
-// it exact.
-__device__ __forceinline__ int slab_boundary(float a0, float da, float ia, int N, float t)
+// in k, so P is false...false true...true).  nia = -1/da estimates k; the two scans
+// make it exact.
+__device__ __forceinline__ int slab_boundary(float a0, float da, float nia, int N, float t)
 {
-    if (N <= 0) return 0;
+    // selects rather than branches: the lanes of a wave differ in direction
     const bool asc = da >= 0.0f;
-    float est;
-    if (da == 0.0f) {
-        est = (a0 >= t) ? 0.0f : (float)N;
-    } else {
-        const float q = (t - a0) * ia;
-        est = asc ? ceilf(q) : floorf(q) + 1.0f;
-    }
-    est = fminf(fmaxf(est, 0.0f), (float)N);
+    const float q = (a0 - t) * nia;  // (t - a0) / da, with nia = -1/da
+    float est = asc ? ceilf(q) : floorf(q) + 1.0f;
+    est = da == 0.0f ? ((a0 >= t) ? 0.0f : (float)N) : est;
+    est = fminf(fmaxf(est, 0.0f), (float)max(N, 0));
     int k = (int)est;
-    auto pred = [&](int kk) {
-        const float a = fmaf((float)kk, da, a0);
-        return asc ? (a >= t) : (a < t);
-    };
+    // P(k) as one comparison: a >= t for da >= 0, a < t otherwise (a is finite)
+    auto pred = [&](int kk) { return (fmaf((float)kk, da, a0) < t) != asc; };
     // one step either way in practice (the estimate's FP32 error is far below a sample);
     // neither unrolled nor interleaved, so that the common exit costs one evaluation of
     // the predicate each
@@ -182,7 +176,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
     // partition the slab's samples and a share ends on a texel boundary whenever l w is
     // an integer
     constexpr float kShare = (float)kSlab / (float)G;
-    int kcarry = slab_boundary(a0, da, ia, N, (float)(srange.x * kSlab - 1));
+    int kcarry = slab_boundary(a0, da, nia, N, (float)(srange.x * kSlab - 1));
     // byte strides of the interleaved tile
     constexpr float kColB = 4.0f * S;
     constexpr float kRowB = 4.0f * S * kPitch;
@@ -195,7 +189,7 @@ __device__ __forceinline__ void fp_bundle(const FpArgs &A, int b, int v, int s0)
         const short2 rr = rows[sl];  // uniform
         const int mlo = rr.x, mhi = rr.y;
         const float tend = lane == G - 1 ? (float)(alo + kSlab) : fmaf((float)(lane + 1), kShare, (float)alo);
-        const int kown = slab_boundary(a0, da, ia, N, tend);
+        const int kown = slab_boundary(a0, da, nia, N, tend);
         // share l - 1's end (kRw lanes below; share 0 of a ray takes the carry instead),
         // and for the next slab's share 0 the end of share G - 1
         int kstart = __shfl(kown, max(wl - kRw, 0));
