@@ -82,6 +82,9 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
     const float *w2row =
         kW && A.w2 ? A.w2 + (arow_ok ? (arow / A.w2div) % A.w2rows : 0) * (long long)A.nC : nullptr;
     const bool vec = ((size_t)A.nC * sizeof(TI)) % 16 == 0 && reinterpret_cast<uintptr_t>(A.in) % 16 == 0;
+    // weight rows 16-byte aligned (FP32 rows of nC, bases from the allocator)
+    const bool wvec = A.nC % 4 == 0 && reinterpret_cast<uintptr_t>(A.w) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(A.w2) % 16 == 0;
     const size_t plane = (size_t)A.Np * A.Kp;
     const __bf16 *hcol = A.hp + (size_t)(c0 + srow) * A.Kp;  // < Np: planes are zero-padded
 
@@ -119,14 +122,39 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 #pragma unroll
                 for (int j = 0; j < 16; ++j) x[j] = (arow_ok && kb + j < A.nC) ? widen(prow[kb + j]) : 0.0f;
             }
-            if constexpr (kW) {
+            // per-sample weights (kW): the product is formed where the split reads it, as
+            // one expression
+            float wt[16];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const bool ok = arow_ok && kb + j < A.nC;
-                    float wt = ok ? 1.0f : 0.0f;
-                    if (wrow) wt *= ok ? wrow[kb + j] : 0.0f;
-                    if (w2row) wt *= ok ? w2row[kb + j] : 0.0f;
-                    x[j] *= wt;
+            for (int j = 0; j < 16; ++j) wt[j] = 1.0f;
+            if constexpr (kW) {
+                if (wvec && arow_ok && kb + 16 <= A.nC) {
+                    // 16-byte loads: a wave's scalar weight loads each touch 32 rows' cache
+                    // lines, and 16 of them per step bound the kernel by its memory
+                    // instruction rate
+                    if (wrow) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float4 u = *reinterpret_cast<const float4 *>(wrow + kb + 4 * q);
+                            wt[4 * q] *= u.x, wt[4 * q + 1] *= u.y, wt[4 * q + 2] *= u.z, wt[4 * q + 3] *= u.w;
+                        }
+                    }
+                    if (w2row) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const float4 u = *reinterpret_cast<const float4 *>(w2row + kb + 4 * q);
+                            wt[4 * q] *= u.x, wt[4 * q + 1] *= u.y, wt[4 * q + 2] *= u.z, wt[4 * q + 3] *= u.w;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const bool ok = arow_ok && kb + j < A.nC;
+                        float w1 = ok ? 1.0f : 0.0f;
+                        if (wrow) w1 *= ok ? wrow[kb + j] : 0.0f;
+                        if (w2row) w1 *= ok ? w2row[kb + j] : 0.0f;
+                        wt[j] = w1;
+                    }
                 }
             }
             bf16x8 h[2], l[2], m[2];
@@ -136,7 +164,7 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
                     h[j >> 3][j & 7] = (__bf16)x[j];
                 } else {
                     __bf16 a, b, c;
-                    split3(x[j], a, b, c);
+                    split3(kW ? x[j] * wt[j] : x[j], a, b, c);
                     h[j >> 3][j & 7] = a, l[j >> 3][j & 7] = b, m[j >> 3][j & 7] = c;
                 }
             }
