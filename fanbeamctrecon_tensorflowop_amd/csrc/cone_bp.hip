@@ -239,6 +239,7 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
     const float cmax = (float)(nC - 1), rmax = (float)(nR - 1);
     const float cfl = (float)max(nC - 2, 0), rfl = (float)max(nR - 2, 0);
     const size_t view_elems = (size_t)nR * nC;
+    const unsigned row_bytes = 4u * (unsigned)nC;
     const float *qvol = A.q + vi * (size_t)A.nViews * view_elems;
 
     // footprints of views vg + m, m < 16: after the 4-lane reduction lane 4m holds the
@@ -287,14 +288,16 @@ __global__ __launch_bounds__(kTx *kTy) void cone_bp_lds_kernel(ConeArgs A)
 #pragma unroll
         for (int m = 0; m < VB; ++m) {
             if (nrow[m] == 0) continue;
+            // rows by 32-bit byte offsets from the view's base (the launch checks that a
+            // view spans less than 4 GiB)
             const float *qv = qvol + (size_t)(v0 + m) * view_elems;
-            const unsigned c = (unsigned)min(max(c_lo[m] + lane, 0), nC - 1);
+            const unsigned c4 = 4u * (unsigned)min(max(c_lo[m] + lane, 0), nC - 1);
 #pragma unroll
             for (int k = 0; k < kRw; ++k) {
                 const int rr = wave + 4 * k;
                 if (rr >= nrow[m]) break;
-                const int r = min(max(r_lo[m] + rr, 0), nR - 1);
-                pv[m][k] = (qv + (size_t)r * nC)[c];
+                const unsigned r = (unsigned)min(max(r_lo[m] + rr, 0), nR - 1);
+                pv[m][k] = load_at(qv, r * row_bytes + c4);
             }
         }
     };
@@ -463,6 +466,8 @@ fbct_status launch_cone_bp(const fbct_cone_plan &p, const ConeDeviceTables &t, c
     A.nzc = (g.nZ + kz - 1) / kz;
     const long long gz = (long long)A.nzc * n;
     if (gz > 65535) return fail(FBCT_ERR_UNSUPPORTED, "cone backprojection: %lld z-groups exceed the grid", gz);
+    if ((unsigned long long)g.nRows * (unsigned long long)g.nCols * 4ull >= (1ull << 32))
+        return fail(FBCT_ERR_UNSUPPORTED, "cone backprojection: a view of %d x %d exceeds 4 GiB", g.nRows, g.nCols);
     const dim3 grid((g.nX + kTx - 1) / kTx, (g.nY + kTy - 1) / kTy, (unsigned)gz);
     if (g.detector == FBCT_DETECTOR_FLAT) launch_nc<true, 0>(kz, grid, stream, A, to);
     else if (p.d.atan_nc == 9) launch_nc<false, 9>(kz, grid, stream, A, to);
