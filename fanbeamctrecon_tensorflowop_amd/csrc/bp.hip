@@ -209,6 +209,7 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
                                   min(ty0 + ((lane & 2) ? kBpTy - 1 : 0), A.nY - 1));
     const float *sino = A.sino + sl * A.sino_stride;  // slice group base
     const float *sino_w = sino + (size_t)(wave * SQ) * A.sino_stride;  // this wave stages slices wave*SQ..
+    const unsigned sbytes = (unsigned)A.sino_stride * 4u;
     float acc[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] = 0.0f;
@@ -230,9 +231,13 @@ __global__ __launch_bounds__(kBpTx *kBpTy) void bp_staged_kernel(BpArgs A, int s
     auto load_batch = [&](int v0, const int (&c_lo)[VB]) {
 #pragma unroll
         for (int m = 0; m < VB; ++m) {
-            const float *src = sino_w + (size_t)min(v0 + m, A.nViews - 1) * A.nCols + min(c_lo[m] + lane, A.nCols - 1);
+            // 32-bit byte offsets from the wave's first slice (the launch stages slice
+            // groups only when four sinograms span less than 4 GiB)
+            const unsigned off =
+                ((unsigned)min(v0 + m, A.nViews - 1) * (unsigned)A.nCols + (unsigned)min(c_lo[m] + lane, A.nCols - 1)) *
+                4u;
 #pragma unroll
-            for (int k = 0; k < SQ; ++k) q[m][k] = src[(size_t)k * A.sino_stride];
+            for (int k = 0; k < SQ; ++k) q[m][k] = load_at(sino_w, off + (unsigned)k * sbytes);
         }
     };
     auto store_batch = [&](int v0, const int (&c_lo)[VB]) {
@@ -721,10 +726,12 @@ void launch_bp_groups(const BpArgs &A, dim3 tiles, int n, hipStream_t stream)
     // slice groups of 16/8/4/2 while they leave enough workgroups to fill the chip, the
     // remainder one slice per group; groups of 4 or more stage their detector columns
     const long long nt = (long long)tiles.x * tiles.y;
+    // the staged kernels address a wave's (up to 4) sinograms by 32-bit byte offsets
+    const bool stage_ok = (unsigned long long)A.sino_stride * 16ull < (1ull << 32);
     int done = 0;
     for (int S : {16, 8, 4, 2, 1}) {
         const int groups = (n - done) / S;
-        if (groups == 0 || (S > 1 && nt * groups < 2048)) continue;
+        if (groups == 0 || (S > 1 && nt * groups < 2048) || (S >= 4 && !stage_ok)) continue;
         const dim3 grid(tiles.x, tiles.y, groups);
         switch (S) {
         case 16: hipLaunchKernelGGL((bp_staged_kernel<kWeighted, 16, kNc>), grid, dim3(kBpTx * kBpTy), 0, stream, A, done); break;
