@@ -239,7 +239,7 @@ fbct_status fbct_fbp_ex(fbct_plan *plan, const float *sino, float *image, int n_
             rw.w2div = 1;
             rw.w2rows = plan->g.nViews;
         }
-        fbct::RampOperand H{t->ramp_planes, plan->g.nCols, plan->d.ramp_np, plan->d.ramp_kp};
+        fbct::RampOperand H{t->ramp_planes, plan->g.nCols, plan->d.ramp_np, plan->d.ramp_kp, plan->d.ramp_lg};
         if ((s = fbct::launch_ramp_gemm(H, sino, FBCT_F32, q, rows, rw, st)) != FBCT_OK) return s;
     }
     return fbct::launch_bp(*plan, *t, q, image, n_slices, true, st);

@@ -110,7 +110,7 @@ void build(fbct_cone_plan &p)
     ConeDerived &d = p.d;
     d.ramp_np = (nC + 127) / 128 * 128;
     d.ramp_kp = (nC + 31) / 32 * 32;
-    toeplitz_planes(p.taps, nC, d.ramp_np, d.ramp_kp, p.ramp_planes);
+    ramp_tap_table(p.taps, nC, d.ramp_np, d.ramp_kp, p.ramp_planes, d.ramp_lg);
     if (flat) {
         p.preweight.resize((size_t)nR * nC);
         for (int r = 0; r < nR; ++r)
@@ -232,7 +232,7 @@ size_t vol_bytes(const fbct_cone_plan *p, int n, int dtype)
 fbct_status run_filter(fbct_cone_plan *p, const ConeDeviceTables &t, const void *proj, fbct_dtype ti, float *q, int n,
                        hipStream_t st)
 {
-    RampOperand H{t.ramp_planes, p->g.nCols, p->d.ramp_np, p->d.ramp_kp};
+    RampOperand H{t.ramp_planes, p->g.nCols, p->d.ramp_np, p->d.ramp_kp, p->d.ramp_lg};
     const int64_t rows = (int64_t)n * p->g.nViews * p->g.nRows;
     RowWeights rw;
     rw.w = t.preweight;  // [nRows][nCols] cosine weights (flat detector)

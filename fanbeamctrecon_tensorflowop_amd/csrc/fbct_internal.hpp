@@ -157,7 +157,7 @@ struct DeviceTables {
     int2 *fov_row = nullptr;       // [nY] pixels i in [x, y] of row j lie in the FOV disk (x > y: none)
     int2 *fov_col = nullptr;       // [nX] pixels j in [x, y] of column i lie in the FOV disk
     double *ramp = nullptr;        // [2*nCols-1]
-    uint16_t *ramp_planes = nullptr;  // [3][Np][Kp] BF16 split planes of the Toeplitz matrix H
+    uint16_t *ramp_planes = nullptr;  // [3][kRampSkew][Lg] BF16 split planes of the skewed ramp taps
     float *parker = nullptr;       // [nViews][nCols] short-scan weights (uploaded on first use)
     float *cosine = nullptr;       // [nCols] FBP cosine pre-weights cos(gamma_c) (uploaded on first use)
 };
@@ -182,7 +182,7 @@ struct Derived {
     float fpa_c_scale;    // 1/dCol
     float fpa_c_off;      // -dLeft/dCol - 1/2
     float fpa_far_x, fpa_far_y;  // 2r - n + 1/2: sample reach beyond the last pixel centre
-    int ramp_np, ramp_kp;         // padded columns / depth of the MFMA filter operand
+    int ramp_np, ramp_kp, ramp_lg;  // padded columns / depth of the MFMA filter operand, tap table row
 };
 
 }  // namespace fbct
@@ -211,7 +211,7 @@ struct fbct_plan {
     // of rows (lo > hi: empty)
     std::vector<int32_t> fov_row, fov_col;
     std::vector<double> ramp;       // 2*nCols-1
-    std::vector<uint16_t> ramp_planes;  // BF16 hi/lo/ll planes of H, [3][Np][Kp]
+    std::vector<uint16_t> ramp_planes;  // BF16 hi/lo/ll planes of the skewed taps, [3][kRampSkew][Lg]
     int64_t fp_samples = 0;
     fbct::Derived d{};
 
@@ -245,7 +245,7 @@ struct ConeDerived {
     double tan_min, tan_max;    //    0 < xi < nCols  <=>  tan_min < t/U < tan_max
     int atan_nc;                // curved: coefficients of the atan correction polynomial (0 = atanf)
     float atan_c[kBpAtanMax];
-    int ramp_np, ramp_kp;
+    int ramp_np, ramp_kp, ramp_lg;
 };
 
 struct ConeDeviceTables {
@@ -253,7 +253,7 @@ struct ConeDeviceTables {
     float2 *view_cs = nullptr;     // [nViews]
     double2 *view_csd = nullptr;   // [nViews]
     float *preweight = nullptr;    // [nRows][nCols] (flat detector)
-    uint16_t *ramp_planes = nullptr;  // [3][Np][Kp]
+    uint16_t *ramp_planes = nullptr;  // [3][kRampSkew][Lg]
     float *parker = nullptr;       // [nViews][nCols] short-scan weights (short_scan plans)
 };
 
@@ -314,8 +314,18 @@ double bf16_value(uint16_t b);
 // atan(z) ~ z + z^3 R(z^2) on |z| <= zmax with nc coefficients of R; false if the fit
 // misses 1e-8 relative error.
 bool fit_atan(double zmax, int nc, float *coef);
-// BF16 hi/lo/ll planes [3][np][kp] of the symmetric Toeplitz matrix H[c][m] = taps[c-m+nC-1].
-void toeplitz_planes(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &planes);
+// Ramp taps for the MFMA filter.  The matrix H[m][c] = h[c - m] = h[m - c] (the taps are
+// symmetric) is Toeplitz: the 8 consecutive entries h[s .. s + 7] that one lane's B
+// fragment holds come from a tap vector.  To read them as one aligned 16-byte load for any
+// s, the table holds kRampSkew = 8 copies shifted by one tap each:
+//     table[pl][r][u] = plane pl (hi, lo, ll) of h[u - r + dlo],   u < lg,
+// zero where |u - r + dlo| > nC - 1; h[s .. s + 7] is row r = (-s) mod 8 from u = s - dlo + r
+// (a multiple of 8).  dlo = -(np + 8) and lg = np + kp + 64 cover every offset an np x kp
+// tiling reads, with the slack of the 8-aligned windows.
+constexpr int kRampSkew = 8;
+constexpr int ramp_table_dlo(int np) { return -(np + 8); }
+constexpr int ramp_table_len(int np, int kp) { return np + kp + 64; }
+void ramp_tap_table(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &table, int &lg);
 
 // hipMalloc + copy of a host vector (at least one element is allocated).
 template <typename T>
@@ -363,10 +373,11 @@ fbct_status launch_fp_batch(const fbct_batch &b, const BatchDevice &bd, const vo
                             void *sino, fbct_dtype to, hipStream_t stream);
 fbct_status launch_fp_adjoint_batch(const fbct_batch &b, const BatchDevice &bd, const void *sino, fbct_dtype ti,
                                     void *img, fbct_dtype to, hipStream_t stream);
-// Toeplitz ramp operand of the MFMA filter: BF16 planes [3][Np][Kp] of H (nC x nC).
+// Ramp operand of the MFMA filter: the skewed tap table [3][kRampSkew][Lg] of H (nC x nC,
+// padded to Np x Kp).
 struct RampOperand {
     const uint16_t *planes;
-    int nC, Np, Kp;
+    int nC, Np, Kp, Lg;
 };
 // rows x nC inputs (FP32 or BF16 storage), optionally times w[r % wrows][c], filtered
 // into FP32 outputs.

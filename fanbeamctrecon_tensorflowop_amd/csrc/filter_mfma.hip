@@ -8,14 +8,16 @@
 // v_mfma_f32_16x16x32_bf16:
 //     hi*hi + hi*lo + lo*hi + lo*lo + hi*ll + ll*hi,
 // which carries the same information as an FP32 product (the dropped terms are below
-// 2^-24 relative).  H's three planes are precomputed per plan (padded with zeros to
-// [Np][Kp]); P is split in registers as it is loaded (optionally multiplied by a
-// per-sample weight first, the cone-beam pre-weight; a BF16 input needs no split).
+// 2^-24 relative).  The taps' three planes are precomputed per plan; P is split in
+// registers as it is loaded (optionally multiplied by a per-sample weight first, the
+// cone-beam pre-weight; a BF16 input needs no split).
 //
 // Tiling: a workgroup of 4 waves computes 128 rows x 128 columns; per K step of 32 it
-// stages the BF16 planes of its P rows (split once per element) and of its H columns
-// (H is symmetric, so the B fragment of column c is row c of H, contiguous in k) in LDS,
-// and each wave multiplies its 64 x 64 quadrant (4 x 4 MFMA tiles).
+// stages the BF16 planes of its P rows (split once per element) and the window of split
+// taps its H columns read (H is Toeplitz and symmetric: the B fragment of column c at k is
+// h[k - c .. k - c + 7], read from skewed copies of the taps so that every fragment is one
+// aligned 16-byte LDS read) in LDS, and each wave multiplies its 64 x 64 quadrant (4 x 4
+// MFMA tiles).
 // Lane layout of v_mfma_f32_16x16x32_bf16: A lane l holds row l%16, k = 8*(l/16)+j;
 // B lane l holds column l%16, k = 8*(l/16)+j; D lane l holds rows 4*(l/16)+j, column l%16.
 
@@ -33,11 +35,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 struct MfmaArgs {
     const void *in;    // TI[rows][nC]
     float *out;        // [rows][nC]
-    const __bf16 *hp;  // [3][Np][Kp] split planes of H
+    const __bf16 *hp;  // [3][kRampSkew][Lg] skewed split taps (fbct_internal.hpp)
     const float *w;    // optional per-sample weights [wrows][nC], row r uses w[r % wrows]
     const float *w2;   // optional second factor [w2rows][nC], row r uses w2[(r / w2div) % w2rows]
     long long rows;
-    int nC, Np, Kp, wrows, w2div, w2rows;
+    int nC, Np, Kp, Lg, wrows, w2div, w2rows;
 };
 
 __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &ll)
@@ -58,6 +60,11 @@ __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uin
 // quadrant (4 x 4 MFMA tiles) from LDS.  Rows of the LDS planes are padded to 40 BF16
 // so the 16 rows a fragment read touches fall in distinct banks.
 constexpr int kBM = 128, kBN = 128, kBK = 32, kKS = 40;
+// Staged taps per plane and copy and K step: the fragments of a K step reach offsets
+// s in [k0 - c0 - 127, k0 - c0 + 31], plus 7 for the copies' shift and 7 for the window's
+// 8-alignment.  176 BF16 = 88 dwords per row (88 / 8 odd): the 16 lanes of a fragment
+// read, 8 copies x 2 windows 16 bytes apart, fall in distinct bank groups.
+constexpr int kTW = kBN + 48;
 
 template <typename TI, bool kW>
 __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
@@ -65,7 +72,7 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
     constexpr bool kExactA = sizeof(TI) == 2 && !kW;
     constexpr int kPA = kExactA ? 1 : 3;  // A planes in use
     __shared__ __bf16 As[kPA][kBM][kKS];
-    __shared__ __bf16 Bs[3][kBN][kKS];
+    __shared__ __bf16 Ts[3][kRampSkew][kTW];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
@@ -85,8 +92,19 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
     // weight rows 16-byte aligned (FP32 rows of nC, bases from the allocator)
     const bool wvec = A.nC % 4 == 0 && reinterpret_cast<uintptr_t>(A.w) % 16 == 0 &&
                       reinterpret_cast<uintptr_t>(A.w2) % 16 == 0;
-    const size_t plane = (size_t)A.Np * A.Kp;
-    const __bf16 *hcol = A.hp + (size_t)(c0 + srow) * A.Kp;  // < Np: planes are zero-padded
+    // B from the tap table: per K step the window of kTW taps (each plane, each skewed
+    // copy) from which column c of H reads h[k0 + k - c], k < 32; the window starts at
+    // u0 + k0, and each lane's B fragment of tile t sits at a fixed offset in it
+    const int dlo = ramp_table_dlo(A.Np);
+    const int u0 = (-c0 - (kBN - 1) - dlo) & ~7;  // >= 8: c0 <= Np - kBN
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    int boff[4];  // Ts element offset of the fragment h[s .. s + 7], s = k0 + fk - c
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int s0 = fk - (c0 + wc * 64 + 16 * t + fr);  // s at k0 = 0
+        const int r = (-s0) & (kRampSkew - 1);               // copy whose row holds it aligned
+        boff[t] = r * kTW + (s0 - dlo + r - u0);             // in [0, kTW - 8], a multiple of 8
+    }
 
     f32x4 acc[4][4];
 #pragma unroll
@@ -177,17 +195,15 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
                 }
             }
         }
-        // ---- stage B: 16 consecutive k of one column of H, three planes
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl)
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-                *reinterpret_cast<bf16x8 *>(&Bs[pl][srow][kh + 8 * q]) =
-                    *reinterpret_cast<const bf16x8 *>(hcol + pl * plane + k0 + kh + 8 * q);
+        // ---- stage B: the tap window of this K step (3 planes x 8 copies x kTW taps)
+        for (int q = tid; q < 3 * kRampSkew * (kTW / 8); q += 256) {
+            const int row = q / (kTW / 8), ch = q % (kTW / 8);  // row = plane * 8 + copy
+            *reinterpret_cast<bf16x8 *>(&Ts[0][0][0] + row * kTW + 8 * ch) =
+                *reinterpret_cast<const bf16x8 *>(A.hp + (size_t)row * A.Lg + (u0 + k0) + 8 * ch);
+        }
         __syncthreads();
 
         // ---- multiply this wave's 64 x 64 quadrant
-        const int fr = lane & 15, fk = 8 * (lane >> 4);
         bf16x8 ah[4], al[4], all[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -200,10 +216,9 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int c = wc * 64 + 16 * t + fr;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bs[0][c][fk]);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bs[1][c][fk]);
-            const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(&Bs[2][c][fk]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Ts[0][0][0] + boff[t]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Ts[1][0][0] + boff[t]);
+            const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(&Ts[2][0][0] + boff[t]);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if constexpr (!kExactA) {
@@ -260,10 +275,11 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     A.nC = H.nC;
     A.Np = H.Np;
     A.Kp = H.Kp;
+    A.Lg = H.Lg;
     A.wrows = rw.w ? rw.wrows : 1;
     A.w2div = rw.w2 ? rw.w2div : 1;
     A.w2rows = rw.w2 ? rw.w2rows : 1;
-    if (H.Np % kBN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC)
+    if (H.Np % kBN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC || H.Lg != ramp_table_len(H.Np, H.Kp))
         return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: operand planes not padded to the tile");
     const long long gx = (rows + kBM - 1) / kBM;
     if (gx > 0x7fffffffLL || gx * ((H.nC + kBN - 1) / kBN) > 0xffffffffLL)
@@ -284,7 +300,7 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
 fbct_status launch_ramp_filter_mfma(const fbct_plan &p, const DeviceTables &t, const float *in, float *out,
                                     int64_t rows, hipStream_t stream)
 {
-    RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp};
+    RampOperand H{t.ramp_planes, p.g.nCols, p.d.ramp_np, p.d.ramp_kp, p.d.ramp_lg};
     return launch_ramp_gemm(H, in, FBCT_F32, out, rows, RowWeights{}, stream);
 }
 

@@ -214,22 +214,25 @@ static void fan_gammas(const fbct_fan_geom &g, std::vector<double> &gamma, doubl
     delta = std::max(std::fabs(dl - sh * dc), std::fabs(dl + (g.nCols - sh) * dc));
 }
 
-void toeplitz_planes(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &planes)
+void ramp_tap_table(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &table, int &lg)
 {
-    planes.assign(static_cast<size_t>(3) * np * kp, 0);
-    const size_t plane = static_cast<size_t>(np) * kp;
-    for (int c = 0; c < nC; ++c)
-        for (int m = 0; m < nC; ++m) {
-            const double x = taps[c - m + nC - 1];
+    lg = ramp_table_len(np, kp);
+    table.assign(static_cast<size_t>(3 * kRampSkew) * lg, 0);
+    const size_t plane = static_cast<size_t>(kRampSkew) * lg;
+    for (int r = 0; r < kRampSkew; ++r)
+        for (int u = 0; u < lg; ++u) {
+            const int d = u - r + ramp_table_dlo(np);
+            if (d < -(nC - 1) || d > nC - 1) continue;
+            const double x = taps[d + nC - 1];
             const uint16_t hi = bf16_rne(static_cast<float>(x));
             const double r1 = x - bf16_value(hi);
             const uint16_t lo = bf16_rne(static_cast<float>(r1));
             const double r2 = r1 - bf16_value(lo);
             const uint16_t ll = bf16_rne(static_cast<float>(r2));
-            const size_t o = static_cast<size_t>(c) * kp + m;
-            planes[o] = hi;
-            planes[plane + o] = lo;
-            planes[2 * plane + o] = ll;
+            const size_t o = static_cast<size_t>(r) * lg + u;
+            table[o] = hi;
+            table[plane + o] = lo;
+            table[2 * plane + o] = ll;
         }
 }
 
@@ -502,12 +505,12 @@ static void build_tables(fbct_plan &p)
         p.ramp[n + nC - 1] = h * kappa;
     }
 
-    // MFMA filter operand: H[m][c] = h[c - m] (symmetric), split into three BF16 planes
-    // hi + lo + ll (each the BF16 rounding of what the previous planes left), padded with
-    // zeros to Np = 64-multiple rows and Kp = 32-multiple columns.
+    // MFMA filter operand: the taps split into three BF16 planes hi + lo + ll (each the
+    // BF16 rounding of what the previous planes left), in skewed copies from which the
+    // filter builds H[m][c] = h[c - m] tile by tile; Np and Kp pad the matrix to the tiles.
     d.ramp_np = (nC + 127) / 128 * 128;
     d.ramp_kp = (nC + 31) / 32 * 32;
-    toeplitz_planes(p.ramp, nC, d.ramp_np, d.ramp_kp, p.ramp_planes);
+    ramp_tap_table(p.ramp, nC, d.ramp_np, d.ramp_kp, p.ramp_planes, d.ramp_lg);
 
     // BP in-range test 0 < xi < nCols <=> g in (dLeft - dShift dCol, dLeft + (nCols - dShift) dCol)
     // (either order when dCol < 0); near the edges it is decided on tangents in FP64.
