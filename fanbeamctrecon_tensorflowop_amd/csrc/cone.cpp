@@ -108,7 +108,7 @@ void build(fbct_cone_plan &p)
         p.taps[n + nC - 1] = h;
     }
     ConeDerived &d = p.d;
-    d.ramp_np = (nC + 127) / 128 * 128;
+    d.ramp_np = (nC + kRampTileN - 1) / kRampTileN * kRampTileN;
     d.ramp_kp = (nC + 31) / 32 * 32;
     ramp_tap_table(p.taps, nC, d.ramp_np, d.ramp_kp, p.ramp_planes, d.ramp_lg);
     if (flat) {

@@ -323,6 +323,7 @@ bool fit_atan(double zmax, int nc, float *coef);
 // (a multiple of 8).  dlo = -(np + 8) and lg = np + kp + 64 cover every offset an np x kp
 // tiling reads, with the slack of the 8-aligned windows.
 constexpr int kRampSkew = 8;
+constexpr int kRampTileN = 256;  // np: a multiple of the widest column tile of the filter
 constexpr int ramp_table_dlo(int np) { return -(np + 8); }
 constexpr int ramp_table_len(int np, int kp) { return np + kp + 64; }
 void ramp_tap_table(const std::vector<double> &taps, int nC, int np, int kp, std::vector<uint16_t> &table, int &lg);

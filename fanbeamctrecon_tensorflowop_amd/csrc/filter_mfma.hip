@@ -59,16 +59,19 @@ __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uin
 // applied first) and of its H columns in LDS, then each wave multiplies its 64 x 64
 // quadrant (4 x 4 MFMA tiles) from LDS.  Rows of the LDS planes are padded to 40 BF16
 // so the 16 rows a fragment read touches fall in distinct banks.
-constexpr int kBM = 128, kBN = 128, kBK = 32, kKS = 40;
-// Staged taps per plane and copy and K step: the fragments of a K step reach offsets
-// s in [k0 - c0 - 127, k0 - c0 + 31], plus 7 for the copies' shift and 7 for the window's
-// 8-alignment.  176 BF16 = 88 dwords per row (88 / 8 odd): the 16 lanes of a fragment
-// read, 8 copies x 2 windows 16 bytes apart, fall in distinct bank groups.
-constexpr int kTW = kBN + 48;
+constexpr int kBM = 128, kBK = 32, kKS = 40;
 
-template <typename TI, bool kW>
+// BN output columns per workgroup (128 or 256; each wave takes BN / 2 of them, NT tiles of
+// 16).  Staged taps per plane and copy and K step: the fragments of a K step reach offsets
+// s in [k0 - c0 - BN + 1, k0 - c0 + 31], plus 7 for the copies' shift and 7 for the
+// window's 8-alignment: BN + 48 BF16 per row, 88 or 152 dwords (an odd multiple of 8, so
+// the 16 lanes of a fragment read, 8 copies x 2 windows 16 bytes apart, fall in distinct
+// bank groups).
+template <typename TI, bool kW, int BN>
 __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 {
+    constexpr int kTW = BN + 48, NT = BN / 32;
+    static_assert((kTW / 2 / 8) % 2 == 1, "tap rows an odd multiple of 8 dwords");
     constexpr bool kExactA = sizeof(TI) == 2 && !kW;
     constexpr int kPA = kExactA ? 1 : 3;  // A planes in use
     __shared__ __bf16 As[kPA][kBM][kKS];
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
     // each XCD takes whole row tiles, their column tiles consecutive (P rows stay in its L2)
     const unsigned lblk = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     const long long r0 = (long long)(lblk / gridDim.y) * kBM;
-    const int c0 = (int)(lblk % gridDim.y) * kBN;
+    const int c0 = (int)(lblk % gridDim.y) * BN;
     // staging assignment: thread -> (row or column tid / 2, 16 k-values at kh)
     const int srow = tid >> 1, kh = (tid & 1) * 16;
     const long long arow = r0 + srow;
@@ -96,21 +99,21 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
     // copy) from which column c of H reads h[k0 + k - c], k < 32; the window starts at
     // u0 + k0, and each lane's B fragment of tile t sits at a fixed offset in it
     const int dlo = ramp_table_dlo(A.Np);
-    const int u0 = (-c0 - (kBN - 1) - dlo) & ~7;  // >= 8: c0 <= Np - kBN
+    const int u0 = (-c0 - (BN - 1) - dlo) & ~7;  // >= 8: c0 <= Np - BN
     const int fr = lane & 15, fk = 8 * (lane >> 4);
-    int boff[4];  // Ts element offset of the fragment h[s .. s + 7], s = k0 + fk - c
+    int boff[NT];  // Ts element offset of the fragment h[s .. s + 7], s = k0 + fk - c
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int s0 = fk - (c0 + wc * 64 + 16 * t + fr);  // s at k0 = 0
+    for (int t = 0; t < NT; ++t) {
+        const int s0 = fk - (c0 + wc * (BN / 2) + 16 * t + fr);  // s at k0 = 0
         const int r = (-s0) & (kRampSkew - 1);               // copy whose row holds it aligned
         boff[t] = r * kTW + (s0 - dlo + r - u0);             // in [0, kTW - 8], a multiple of 8
     }
 
-    f32x4 acc[4][4];
+    f32x4 acc[4][NT];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[i][t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < NT; ++t) acc[i][t] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int k0 = 0; k0 < A.Kp; k0 += kBK) {
         // ---- stage A: 16 consecutive k of one row, weighted and split
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
             }
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NT; ++t) {
             const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Ts[0][0][0] + boff[t]);
             const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Ts[1][0][0] + boff[t]);
             const bf16x8 bll = *reinterpret_cast<const bf16x8 *>(&Ts[2][0][0] + boff[t]);
@@ -236,8 +239,8 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 
     // ---- store: D lane l holds rows 4 (l / 16) + j, column l % 16 of each 16 x 16 tile
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int col = c0 + wc * 64 + 16 * t + (lane & 15);
+    for (int t = 0; t < NT; ++t) {
+        const int col = c0 + wc * (BN / 2) + 16 * t + (lane & 15);
         if (col >= A.nC) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -251,9 +254,10 @@ __global__ __launch_bounds__(256) void ramp_gemm_kernel(MfmaArgs A)
 }
 
 template <typename TI, bool kW>
-void launch_gemm_p(dim3 grid, hipStream_t stream, const MfmaArgs &A)
+void launch_gemm_p(int bn, dim3 grid, hipStream_t stream, const MfmaArgs &A)
 {
-    hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW>), grid, dim3(256), 0, stream, A);
+    if (bn == 256) hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, 256>), grid, dim3(256), 0, stream, A);
+    else hipLaunchKernelGGL((ramp_gemm_kernel<TI, kW, 128>), grid, dim3(256), 0, stream, A);
 }
 
 }  // namespace
@@ -279,18 +283,21 @@ fbct_status launch_ramp_gemm(const RampOperand &H, const void *in, fbct_dtype ti
     A.wrows = rw.w ? rw.wrows : 1;
     A.w2div = rw.w2 ? rw.w2div : 1;
     A.w2rows = rw.w2 ? rw.w2rows : 1;
-    if (H.Np % kBN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC || H.Lg != ramp_table_len(H.Np, H.Kp))
+    if (H.Np % kRampTileN != 0 || H.Np < H.nC || H.Kp % kBK != 0 || H.Kp < H.nC || H.Lg != ramp_table_len(H.Np, H.Kp))
         return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: operand planes not padded to the tile");
+    // 256-column tiles (the input rows are read by half as many workgroups) unless they
+    // pad the columns further than 128-column tiles do
+    const int bn = (H.nC + 255) / 256 * 256 == (H.nC + 127) / 128 * 128 ? 256 : 128;
     const long long gx = (rows + kBM - 1) / kBM;
-    if (gx > 0x7fffffffLL || gx * ((H.nC + kBN - 1) / kBN) > 0xffffffffLL)
+    if (gx > 0x7fffffffLL || gx * ((H.nC + bn - 1) / bn) > 0xffffffffLL)
         return fail(FBCT_ERR_UNSUPPORTED, "ramp filter: too many rows");
-    dim3 grid((unsigned)gx, (H.nC + kBN - 1) / kBN);
+    dim3 grid((unsigned)gx, (H.nC + bn - 1) / bn);
     if (ti == FBCT_BF16) {
-        if (w) launch_gemm_p<uint16_t, true>(grid, stream, A);
-        else launch_gemm_p<uint16_t, false>(grid, stream, A);
+        if (w) launch_gemm_p<uint16_t, true>(bn, grid, stream, A);
+        else launch_gemm_p<uint16_t, false>(bn, grid, stream, A);
     } else {
-        if (w) launch_gemm_p<float, true>(grid, stream, A);
-        else launch_gemm_p<float, false>(grid, stream, A);
+        if (w) launch_gemm_p<float, true>(bn, grid, stream, A);
+        else launch_gemm_p<float, false>(bn, grid, stream, A);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(FBCT_ERR_DEVICE, "ramp_gemm_kernel launch: %s", hipGetErrorString(e));

@@ -508,7 +508,7 @@ static void build_tables(fbct_plan &p)
     // MFMA filter operand: the taps split into three BF16 planes hi + lo + ll (each the
     // BF16 rounding of what the previous planes left), in skewed copies from which the
     // filter builds H[m][c] = h[c - m] tile by tile; Np and Kp pad the matrix to the tiles.
-    d.ramp_np = (nC + 127) / 128 * 128;
+    d.ramp_np = (nC + kRampTileN - 1) / kRampTileN * kRampTileN;
     d.ramp_kp = (nC + 31) / 32 * 32;
     ramp_tap_table(p.ramp, nC, d.ramp_np, d.ramp_kp, p.ramp_planes, d.ramp_lg);
 
