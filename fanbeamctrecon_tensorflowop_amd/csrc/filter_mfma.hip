@@ -12,12 +12,12 @@
 // registers as it is loaded (optionally multiplied by a per-sample weight first, the
 // cone-beam pre-weight; a BF16 input needs no split).
 //
-// Tiling: a workgroup of 4 waves computes 128 rows x 128 columns; per K step of 32 it
+// Tiling: a workgroup of 4 waves computes 128 rows x 128 (or 256) columns; per K step of 32 it
 // stages the BF16 planes of its P rows (split once per element) and the window of split
 // taps its H columns read (H is Toeplitz and symmetric: the B fragment of column c at k is
 // h[k - c .. k - c + 7], read from skewed copies of the taps so that every fragment is one
-// aligned 16-byte LDS read) in LDS, and each wave multiplies its 64 x 64 quadrant (4 x 4
-// MFMA tiles).
+// aligned 16-byte LDS read) in LDS, and each wave multiplies its part, 64 rows x 64 (or
+// 128) columns of 16 x 16 MFMA tiles.
 // Lane layout of v_mfma_f32_16x16x32_bf16: A lane l holds row l%16, k = 8*(l/16)+j;
 // B lane l holds column l%16, k = 8*(l/16)+j; D lane l holds rows 4*(l/16)+j, column l%16.
 
@@ -54,11 +54,11 @@ __device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &lo, __bf16 &
 __device__ __forceinline__ float widen(float x) { return x; }
 __device__ __forceinline__ float widen(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// LDS-tiled version.  A workgroup (4 waves) computes 128 rows x 128 columns; per K step
-// of 32 it stages the three BF16 planes of its P rows (split once per element, weights
-// applied first) and of its H columns in LDS, then each wave multiplies its 64 x 64
-// quadrant (4 x 4 MFMA tiles) from LDS.  Rows of the LDS planes are padded to 40 BF16
-// so the 16 rows a fragment read touches fall in distinct banks.
+// A workgroup (4 waves) computes 128 rows x BN columns; per K step of 32 it stages the
+// BF16 planes of its P rows (split once per element, weights applied first) and the tap
+// window of its H columns in LDS, then each wave multiplies its 64 x BN/2 part from LDS.
+// Rows of the P planes are padded to 40 BF16 so the 16 rows a fragment read touches fall
+// in distinct banks.
 constexpr int kBM = 128, kBK = 32, kKS = 40;
 
 // BN output columns per workgroup (128 or 256; each wave takes BN / 2 of them, NT tiles of
