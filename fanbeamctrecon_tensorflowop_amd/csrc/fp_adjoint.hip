@@ -109,6 +109,59 @@ __device__ __forceinline__ void candidate_rays(const FpaArgs &A, float2 cs, floa
     chi = (int)fminf(floorf(cf + hw), (float)(A.nCols - 1));
 }
 
+// Candidate rays of the cells of one 4 x 4 block in one view, from a linear model of the
+// column coordinate about the block centre (xc, yc) (mm): cf(p) = c_scale atan(t/U) +
+// c_off is c_scale times the polar angle of p about the source, whose Hessian has norm
+// 1/r^2 (r the distance to the source, r >= U).  Over the block's cell centres, within
+// h = 1.5 dx of the centre along each axis, the linear model is therefore off by at most
+// |c_scale| h^2 / Umin^2, Umin = U - (|cos b| + |sin b|) h the least U over the block; the
+// half-width adds that to the cell bound of candidate_rays evaluated at Umin (its
+// largest z), so every cell's range holds every column the cell's own evaluation admits.
+// One lane evaluates a view's model; the 16 lanes of the block read it with shuffles.
+struct BlockCand {
+    float cf, gx, gy, hw;  // centre column, d cf / dx, d cf / dy (per mm), half-width
+};
+
+__device__ __forceinline__ BlockCand block_candidates(const FpaArgs &A, float2 cs, float xc, float yc, float rho)
+{
+    const float U = fmaf(-yc, cs.y, fmaf(-xc, cs.x, A.R));
+    const float t = fmaf(xc, cs.y, -yc * cs.x);
+    const float rU = __builtin_amdgcn_rcpf(U);
+    BlockCand b;
+    b.cf = fmaf(cand_atan(t * rU), A.c_scale, A.c_off);
+    // d atan(t/U) = (U dt - t dU) / (U^2 + t^2) with dt = (sin b, -cos b) . dp and
+    // dU = -(cos b, sin b) . dp
+    const float k = A.c_scale * __builtin_amdgcn_rcpf(fmaf(U, U, t * t));
+    b.gx = k * fmaf(U, cs.y, t * cs.x);
+    b.gy = k * fmaf(t, cs.y, -U * cs.x);
+    const float h = 1.5f * A.dx;
+    const float Umin = U - (fabsf(cs.x) + fabsf(cs.y)) * h;
+    const float rUm = __builtin_amdgcn_rcpf(Umin);
+    const float z = rho * A.dx * rUm;
+    b.hw = (Umin > 0.0f && z < 0.9f)
+               ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs, fmaf(kCandAtanErr, A.c_abs, fmaf(A.c_abs * h * h, rUm * rUm, 1e-3f)))
+               : 1e9f;
+    return b;
+}
+
+// The cell's candidate range from its block's model (dxl, dyl: cell centre minus block
+// centre, mm).
+__device__ __forceinline__ void cell_candidates(const FpaArgs &A, const BlockCand &b, float dxl, float dyl, int &clo,
+                                                int &chi)
+{
+    const float cf = fmaf(b.gx, dxl, fmaf(b.gy, dyl, b.cf));
+    clo = (int)fmaxf(ceilf(cf - b.hw), 0.0f);
+    chi = (int)fminf(floorf(cf + b.hw), (float)(A.nCols - 1));
+}
+
+// The model of view j of a group of 16 (lane (lane & ~15) | j holds it): every lane of
+// the wave takes part.
+__device__ __forceinline__ BlockCand block_model_of(const BlockCand &mine, int j)
+{
+    const int src = ((int)threadIdx.x & 48) | j;
+    return BlockCand{__shfl(mine.cf, src), __shfl(mine.gx, src), __shfl(mine.gy, src), __shfl(mine.hw, src)};
+}
+
 // W_c(i, j): total bilinear weight pixel (fi, fj) receives from the N samples of ray
 // (u0, w0, du, dw) = g, evaluated at the FP32 sample positions; inv = (1/du, 1/dw).
 template <bool kBorder>
@@ -527,9 +580,6 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
     tile_cell(lx, ly);
     const int ci = tx * kCellOut - 1 + lx, cj = ty * kCellOut - 1 + ly;  // >= -1
     const float fci = (float)ci, fcj = (float)cj;
-    // cell centre in mm: texel coordinate ci + 1/2 (texel u sits at (u + 1/2 - r) dx)
-    const float x = (fci + 1.0f - A.r) * A.dx;
-    const float y = (fcj + 1.0f - A.r) * A.dx;
     const TI *sino[S];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -540,23 +590,40 @@ __device__ __forceinline__ void cell_tile(const FpaArgs &A, int tx, int ty, int 
 #pragma unroll
         for (int s = 0; s < S; ++s) acc[q][s] = 0.0f;
 
-    if (ci <= A.nX - 1 && cj <= A.nY - 1 && cell_in_fov(A, fci, fcj)) {
-        for (int v = 0; v < A.nViews; ++v) {
-            int clo, chi;
-            candidate_rays(A, A.cs[v], x, y, kCellRho, clo, chi);
-            // 32-bit element offsets from the tables' bases (a plan's tables hold fewer than
-            // 2^26 rays)
-            const unsigned rbase = (unsigned)v * (unsigned)A.nCols;
-            for (int c = clo; c <= chi; ++c) {
-                const unsigned rb = rbase + (unsigned)c;
-                const float N = (float)load_at(A.ray_n, rb * 4u);
-                const float4 g = load_at(A.ray_geo, rb * 16u);
-                const float2 inv = load_at(A.ray_inv, rb * 8u);
-                const unsigned boff = rb * (unsigned)sizeof(TI);
-                float sv[S];
+    const bool active = ci <= A.nX - 1 && cj <= A.nY - 1 && cell_in_fov(A, fci, fcj);
+    // block of 4 x 4 cells of this lane's group of 16: centre and this cell's offset (mm);
+    // cell ci's centre is texel coordinate ci + 1/2, at (ci + 1 - r) dx
+    const float bx = (float)(tx * kCellOut - 1 + (lx & ~3)) + 2.5f - A.r;  // centre, texel units
+    const float by = (float)(ty * kCellOut - 1 + (ly & ~3)) + 2.5f - A.r;
+    const float dxl = ((float)(lx & 3) - 1.5f) * A.dx, dyl = ((float)(ly & 3) - 1.5f) * A.dx;
+    // all lanes run the view loop (the models are shared by shuffles); inactive cells skip
+    // the rays, and a wave without an active cell skips it all
+    if (__any(active)) {
+        for (int vb = 0; vb < A.nViews; vb += 16) {
+            // lane j of each group of 16 evaluates the model of view vb + j
+            const int vj = min(vb + ((int)threadIdx.x & 15), A.nViews - 1);
+            const BlockCand mine = block_candidates(A, A.cs[vj], bx * A.dx, by * A.dx, kCellRho);
+            const int nv = min(16, A.nViews - vb);
+            for (int j = 0; j < nv; ++j) {
+                const BlockCand bm = block_model_of(mine, j);
+                if (!active) continue;
+                const int v = vb + j;
+                int clo, chi;
+                cell_candidates(A, bm, dxl, dyl, clo, chi);
+                // 32-bit element offsets from the tables' bases (a plan's tables hold fewer than
+                // 2^26 rays)
+                const unsigned rbase = (unsigned)v * (unsigned)A.nCols;
+                for (int c = clo; c <= chi; ++c) {
+                    const unsigned rb = rbase + (unsigned)c;
+                    const float N = (float)load_at(A.ray_n, rb * 4u);
+                    const float4 g = load_at(A.ray_geo, rb * 16u);
+                    const float2 inv = load_at(A.ray_inv, rb * 8u);
+                    const unsigned boff = rb * (unsigned)sizeof(TI);
+                    float sv[S];
 #pragma unroll
-                for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
-                cell_accumulate<S>(g, inv, N, sv, fci, fcj, acc);
+                    for (int s = 0; s < S; ++s) sv[s] = load_f_at(sino[s], boff);
+                    cell_accumulate<S>(g, inv, N, sv, fci, fcj, acc);
+                }
             }
         }
     }
@@ -737,6 +804,8 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
             load_batch(v0 + VB, c_nx);
         }
         if (!active) continue;
+        // (each cell evaluates its own candidates here: the shared block model of the
+        // unstaged tile ran 3.5% slower in this kernel, which holds more registers)
 #pragma unroll 1
         for (int m = 0; m < VB; ++m) {
             const int v = v0 + m;
