@@ -328,3 +328,41 @@ def test_candidate_atan_polynomial_bound():
     y = (q * r).astype(np.float32)
     err = np.abs(y.astype(np.float64) - np.arctan(q.astype(np.float64))).max()
     assert err < 0.5 * bound, err
+
+
+def test_block_candidate_linear_model_bound():
+    """The unstaged FP^T tile takes a 4 x 4 cell block's candidate columns from a linear
+    model of the fan angle about the block centre and widens the half-width by h^2 / Umin^2
+    (radians; h = 1.5 texels, Umin = U - (|cos b| + |sin b|) h the least U over the block).
+    The fan angle is the polar angle about the source, whose Hessian has norm 1/r^2 <=
+    1/U^2.  Check in FP64, over random views, block centres in the FOV and geometries down
+    to a source close to the FOV edge, that the model's error at the block's 16 cell
+    centres stays within that bound (a bound too small would drop candidate rays)."""
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(400):
+        R = rng.uniform(150.0, 800.0)
+        rfov = rng.uniform(0.2, 0.8) * R
+        dx = 2.0 * rfov / rng.choice([256, 384, 512, 768, 1024])
+        b = rng.uniform(0.0, 2.0 * np.pi)
+        c, s = np.cos(b), np.sin(b)
+        rad = rfov * np.sqrt(rng.uniform(0.0, 1.0))
+        phi = rng.uniform(0.0, 2.0 * np.pi)
+        xc, yc = rad * np.cos(phi), rad * np.sin(phi)
+        U = R - xc * c - yc * s
+        t = xc * s - yc * c
+        h = 1.5 * dx
+        umin = U - (abs(c) + abs(s)) * h
+        if umin <= 0.0:
+            continue
+        g0 = np.arctan(t / U)
+        gx = (U * s + t * c) / (U * U + t * t)
+        gy = (t * s - U * c) / (U * U + t * t)
+        off = (np.arange(4) - 1.5) * dx
+        px, py = np.meshgrid(xc + off, yc + off)
+        Up = R - px * c - py * s
+        tp = px * s - py * c
+        err = np.abs(np.arctan(tp / Up) - (g0 + gx * (px - xc) + gy * (py - yc))).max()
+        bound = h * h / (umin * umin)
+        worst = max(worst, err / bound)
+    assert 0.0 < worst <= 1.0, worst
