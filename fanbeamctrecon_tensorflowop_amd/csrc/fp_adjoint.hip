@@ -137,10 +137,25 @@ __device__ __forceinline__ BlockCand block_candidates(const FpaArgs &A, float2 c
     const float h = 1.5f * A.dx;
     const float Umin = U - (fabsf(cs.x) + fabsf(cs.y)) * h;
     const float rUm = __builtin_amdgcn_rcpf(Umin);
-    const float z = rho * A.dx * rUm;
-    b.hw = (Umin > 0.0f && z < 0.9f)
-               ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs, fmaf(kCandAtanErr, A.c_abs, fmaf(A.c_abs * h * h, rUm * rUm, 1e-3f)))
-               : 1e9f;
+    auto half_width = [&](float rs) {  // for a support of radius rs (texels) about each cell centre
+        const float z = rs * A.dx * rUm;
+        return (Umin > 0.0f && z < 0.9f)
+                   ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs,
+                          fmaf(kCandAtanErr, A.c_abs, fmaf(A.c_abs * h * h, rUm * rUm, 1e-3f)))
+                   : 1e9f;
+    };
+    // A line of unit direction d meets a square cell iff it passes within (|d_x| + |d_y|)/2
+    // texels of its centre, which is below rho = 1/sqrt(2) except along the diagonals.  The
+    // direction from the source to the block centre is (t sin b - U cos b, -U sin b - t cos b)
+    // / sqrt(U^2 + t^2); a candidate ray's direction differs from it by at most 1.415 h / Umin
+    // (its cell's place in the block) plus hw / |c_scale| (its column, within the disk's
+    // half-width hw), and (|d_x| + |d_y|)/2 changes by at most 0.7072 per radian; 0.01 texel
+    // covers the FP32 sample positions.  Only rays without samples in the cell are dropped.
+    const float hw_disk = half_width(rho);
+    const float ax = fabsf(fmaf(t, cs.y, -U * cs.x)), ay = fabsf(fmaf(U, cs.y, t * cs.x));
+    const float spread = fmaf(1.415f * h, rUm, hw_disk * __builtin_amdgcn_rcpf(A.c_abs));
+    const float rs = fminf(rho, fmaf(0.5f * (ax + ay), rsqrtf(fmaf(U, U, t * t)), fmaf(0.7072f, spread, 0.01f)));
+    b.hw = half_width(rs);
     return b;
 }
 

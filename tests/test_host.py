@@ -366,3 +366,56 @@ def test_block_candidate_linear_model_bound():
         bound = h * h / (umin * umin)
         worst = max(worst, err / bound)
     assert 0.0 < worst <= 1.0, worst
+
+
+def test_block_candidate_square_support():
+    """The block model's half-width uses the cell's square support: a line of unit direction
+    d meets a square cell iff it passes within (|d_x| + |d_y|)/2 texels of its centre,
+    taken at the source-to-block-centre direction and widened by 0.7072 per radian of
+    direction spread (1.415 h / Umin over the block, plus the disk window's half-width for
+    the ray), capped at the half diagonal.  Check in FP64, without the FP32 margins, that
+    every cell's exact fan-angle range (the extremes over its four corners) lies inside the
+    window the model gives that cell, over random views, block centres and geometries."""
+    rng = np.random.default_rng(11)
+    atan_err = 2.0e-6
+    worst, narrower = 0.0, []
+    for _ in range(600):
+        R = rng.uniform(150.0, 800.0)
+        rfov = rng.uniform(0.2, 0.8) * R
+        dx = 2.0 * rfov / rng.choice([256, 384, 512, 768, 1024])
+        b = rng.uniform(0.0, 2.0 * np.pi)
+        c, s = np.cos(b), np.sin(b)
+        rad = rfov * np.sqrt(rng.uniform(0.0, 1.0))
+        phi = rng.uniform(0.0, 2.0 * np.pi)
+        xc, yc = rad * np.cos(phi), rad * np.sin(phi)
+        U = R - xc * c - yc * s
+        t = xc * s - yc * c
+        h = 1.5 * dx
+        umin = U - (abs(c) + abs(s)) * h
+        if umin <= 0.0:
+            continue
+
+        def half_width(rs):  # radians
+            z = rs * dx / umin
+            return z * (1.0 + z * z) + atan_err + h * h / (umin * umin) if z < 0.9 else np.inf
+
+        hw_disk = half_width(np.sqrt(0.5))
+        ax, ay = abs(t * s - U * c), abs(U * s + t * c)
+        rs = min(np.sqrt(0.5), 0.5 * (ax + ay) / np.hypot(U, t) + 0.7072 * (1.415 * h / umin + hw_disk))
+        hw = half_width(rs)
+        if not np.isfinite(hw):
+            continue
+        narrower.append(hw / hw_disk)
+        g0 = np.arctan(t / U)
+        gx = (U * s + t * c) / (U * U + t * t)
+        gy = (t * s - U * c) / (U * U + t * t)
+        off = (np.arange(4) - 1.5) * dx
+        px, py = np.meshgrid(xc + off, yc + off)
+        lin = g0 + gx * (px - xc) + gy * (py - yc)
+        for ex in (-0.5, 0.5):
+            for ey in (-0.5, 0.5):
+                qx, qy = px + ex * dx, py + ey * dx
+                ang = np.arctan((qx * s - qy * c) / (R - qx * c - qy * s))
+                worst = max(worst, np.abs(ang - lin).max() / hw)
+    assert 0.0 < worst <= 1.0, worst
+    assert np.mean(narrower) < 0.97, np.mean(narrower)
