@@ -112,12 +112,14 @@ __device__ __forceinline__ void candidate_rays(const FpaArgs &A, float2 cs, floa
 // Candidate rays of the cells of one 4 x 4 block in one view, from a linear model of the
 // column coordinate about the block centre (xc, yc) (mm): cf(p) = c_scale atan(t/U) +
 // c_off is c_scale times the polar angle of p about the source, whose Hessian has norm
-// 1/r^2 (r the distance to the source, r >= U).  Over the block's cell centres, within
-// h = 1.5 dx of the centre along each axis, the linear model is therefore off by at most
-// |c_scale| h^2 / Umin^2, Umin = U - (|cos b| + |sin b|) h the least U over the block; the
-// half-width adds that to the cell bound of candidate_rays evaluated at Umin (its
-// largest z), so every cell's range holds every column the cell's own evaluation admits.
-// One lane evaluates a view's model; the 16 lanes of the block read it with shuffles.
+// 1/r^2 (r the distance to the source).  Over the block's cell centres, within h = 1.5 dx
+// of the centre along each axis, the linear model is therefore off by at most
+// |c_scale| h^2 / Rmin^2, Rmin the least r over the block.  A ray at fan-angle offset g
+// from a cell centre passes r sin|g| from it, so the rays that meet the cell lie within
+// asin(rs dx / Rmin) <= z (1 + z^2), z = rs dx / Rmin, of its fan angle (rs the cell's
+// support radius, below); the half-width adds the model's error and the atan bound to
+// that, so every cell's range holds every column whose ray meets the cell.  One lane
+// evaluates a view's model; the 16 lanes of the block read it with shuffles.
 struct BlockCand {
     float cf, gx, gy, hw;  // centre column, d cf / dx, d cf / dy (per mm), half-width
 };
@@ -136,25 +138,30 @@ __device__ __forceinline__ BlockCand block_candidates(const FpaArgs &A, float2 c
     b.gy = k * fmaf(t, cs.y, -U * cs.x);
     const float h = 1.5f * A.dx;
     const float Umin = U - (fabsf(cs.x) + fabsf(cs.y)) * h;
-    const float rUm = __builtin_amdgcn_rcpf(Umin);
+    // every point of the block is at least Rmin from the source: its U is at least Umin,
+    // and it lies within 1.415 h of the block centre, at sqrt(U^2 + t^2) from the source
+    const float q = fmaf(U, U, t * t);
+    const float rL = rsqrtf(q);
+    const float rRm = __builtin_amdgcn_rcpf(fmaxf(Umin, fmaf(q, rL, -1.415f * h)));
     auto half_width = [&](float rs) {  // for a support of radius rs (texels) about each cell centre
-        const float z = rs * A.dx * rUm;
+        const float z = rs * A.dx * rRm;
         return (Umin > 0.0f && z < 0.9f)
                    ? fmaf(z * fmaf(z, z, 1.0f), A.c_abs,
-                          fmaf(kCandAtanErr, A.c_abs, fmaf(A.c_abs * h * h, rUm * rUm, 1e-3f)))
+                          fmaf(kCandAtanErr, A.c_abs, fmaf(A.c_abs * h * h, rRm * rRm, 1e-3f)))
                    : 1e9f;
     };
     // A line of unit direction d meets a square cell iff it passes within (|d_x| + |d_y|)/2
     // texels of its centre, which is below rho = 1/sqrt(2) except along the diagonals.  The
     // direction from the source to the block centre is (t sin b - U cos b, -U sin b - t cos b)
-    // / sqrt(U^2 + t^2); a candidate ray's direction differs from it by at most 1.415 h / Umin
+    // / sqrt(U^2 + t^2); a candidate ray's direction differs from it by at most 1.415 h / Rmin
     // (its cell's place in the block) plus hw / |c_scale| (its column, within the disk's
-    // half-width hw), and (|d_x| + |d_y|)/2 changes by at most 0.7072 per radian; 0.01 texel
-    // covers the FP32 sample positions.  Only rays without samples in the cell are dropped.
+    // half-width hw), and (|d_x| + |d_y|)/2 changes by at most 0.7072 per radian; 0.004 texel
+    // covers the FP32 sample positions (within 3e-4 texel of the line).  Only rays without
+    // samples in the cell are dropped.
     const float hw_disk = half_width(rho);
     const float ax = fabsf(fmaf(t, cs.y, -U * cs.x)), ay = fabsf(fmaf(U, cs.y, t * cs.x));
-    const float spread = fmaf(1.415f * h, rUm, hw_disk * __builtin_amdgcn_rcpf(A.c_abs));
-    const float rs = fminf(rho, fmaf(0.5f * (ax + ay), rsqrtf(fmaf(U, U, t * t)), fmaf(0.7072f, spread, 0.01f)));
+    const float spread = fmaf(1.415f * h, rRm, hw_disk * __builtin_amdgcn_rcpf(A.c_abs));
+    const float rs = fminf(rho, fmaf(0.5f * (ax + ay), rL, fmaf(0.7072f, spread, 0.004f)));
     b.hw = half_width(rs);
     return b;
 }

@@ -372,8 +372,9 @@ def test_block_candidate_square_support():
     """The block model's half-width uses the cell's square support: a line of unit direction
     d meets a square cell iff it passes within (|d_x| + |d_y|)/2 texels of its centre,
     taken at the source-to-block-centre direction and widened by 0.7072 per radian of
-    direction spread (1.415 h / Umin over the block, plus the disk window's half-width for
-    the ray), capped at the half diagonal.  Check in FP64, without the FP32 margins, that
+    direction spread (1.415 h / Rmin over the block, plus the disk window's half-width for
+    the ray), capped at the half diagonal; Rmin = max(Umin, sqrt(U^2 + t^2) - 1.415 h)
+    bounds the distance to the source over the block.  Check in FP64, without the FP32 margins, that
     every cell's exact fan-angle range (the extremes over its four corners) lies inside the
     window the model gives that cell, over random views, block centres and geometries."""
     rng = np.random.default_rng(11)
@@ -394,14 +395,15 @@ def test_block_candidate_square_support():
         umin = U - (abs(c) + abs(s)) * h
         if umin <= 0.0:
             continue
+        rmin = max(umin, np.hypot(U, t) - 1.415 * h)  # least distance to the source
 
         def half_width(rs):  # radians
-            z = rs * dx / umin
-            return z * (1.0 + z * z) + atan_err + h * h / (umin * umin) if z < 0.9 else np.inf
+            z = rs * dx / rmin
+            return z * (1.0 + z * z) + atan_err + h * h / (rmin * rmin) if z < 0.9 else np.inf
 
         hw_disk = half_width(np.sqrt(0.5))
         ax, ay = abs(t * s - U * c), abs(U * s + t * c)
-        rs = min(np.sqrt(0.5), 0.5 * (ax + ay) / np.hypot(U, t) + 0.7072 * (1.415 * h / umin + hw_disk))
+        rs = min(np.sqrt(0.5), 0.5 * (ax + ay) / np.hypot(U, t) + 0.7072 * (1.415 * h / rmin + hw_disk))
         hw = half_width(rs)
         if not np.isfinite(hw):
             continue
