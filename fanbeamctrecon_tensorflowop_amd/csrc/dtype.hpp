@@ -33,6 +33,17 @@ __device__ __forceinline__ float load_f_at(const bf16_t *p, unsigned byte_off)
     return __uint_as_float(static_cast<unsigned>(b) << 16);
 }
 
+// A loaded storage element widened to FP32 (exact).  Kept apart from the load where a
+// prefetched value waits in registers: widening at the load would wait for it there.
+__device__ __forceinline__ float widen(float v)
+{
+    return v;
+}
+__device__ __forceinline__ float widen(bf16_t v)
+{
+    return __uint_as_float(static_cast<unsigned>(v) << 16);
+}
+
 // *(base + byte_off): a 32-bit offset from the base (a scalar base takes it as the lane
 // offset of the load)
 template <typename T>

@@ -754,7 +754,7 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
         for (int m = 0; m < VB; ++m)
             c_lo[m] = min(max(__builtin_amdgcn_readlane(wlo, 4 * m) - 1, 0), max(A.nCols - kStageCols, 0));
     };
-    float pv[VB][SQ];             // next batch: this wave's values of lane's column
+    TI pv[VB][SQ];                // next batch: this wave's values of lane's column (storage bits)
     float4 pg[kRecViews];         // its ray records: geo
     float2 pi[kRecViews];         //   1 / steps
     int pn[kRecViews];            //   samples
@@ -764,11 +764,11 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
             const int c = min(c_lo[m] + lane, A.nCols - 1);
             const unsigned boff = (unsigned)((size_t)min(v0 + m, A.nViews - 1) * A.nCols + c) * (unsigned)sizeof(TI);
             if constexpr (S == 1) {
-                if (wave == 0) pv[m][0] = load_f_at(gbase, boff);
+                if (wave == 0) pv[m][0] = load_at(gbase, boff);
             } else {
 #pragma unroll
                 for (int q = 0; q < SQ; ++q)
-                    pv[m][q] = load_f_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
+                    pv[m][q] = load_at(gbase, (unsigned)min(wave * SQ + q, last) * sbytes + boff);
             }
         }
 #pragma unroll
@@ -787,19 +787,19 @@ __device__ __forceinline__ void cell_staged_tile(const FpaArgs &A, CellShared<S,
         for (int m = 0; m < VB; ++m) {
             if (v0 + m >= A.nViews || c_lo[m] + lane >= A.nCols) continue;
             if constexpr (S == 1) {
-                if (wave == 0) st.val[m][lane][0] = pv[m][0];
+                if (wave == 0) st.val[m][lane][0] = widen(pv[m][0]);
             } else {
                 float *dst = &st.val[m][lane][wave * SQ];  // this wave's SQ slots of the column
                 if constexpr (SQ % 4 == 0) {
 #pragma unroll
                     for (int q = 0; q < SQ; q += 4)
-                        *reinterpret_cast<float4 *>(dst + q) =
-                            make_float4(pv[m][q], pv[m][q + 1], pv[m][q + 2], pv[m][q + 3]);
+                        *reinterpret_cast<float4 *>(dst + q) = make_float4(
+                            widen(pv[m][q]), widen(pv[m][q + 1]), widen(pv[m][q + 2]), widen(pv[m][q + 3]));
                 } else if constexpr (SQ == 2) {
-                    *reinterpret_cast<float2 *>(dst) = make_float2(pv[m][0], pv[m][1]);
+                    *reinterpret_cast<float2 *>(dst) = make_float2(widen(pv[m][0]), widen(pv[m][1]));
                 } else {
 #pragma unroll
-                    for (int q = 0; q < SQ; ++q) dst[q] = pv[m][q];
+                    for (int q = 0; q < SQ; ++q) dst[q] = widen(pv[m][q]);
                 }
             }
         }
